@@ -1,0 +1,132 @@
+"""ctypes binding of `libaleph_hip.so` (declarations in include/aleph_hip.h).
+
+Loading fails loudly when the library has not been built: there is no
+CPU fallback behind this module.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libaleph_hip.so"
+
+u32, u64, i32, f64 = C.c_uint32, C.c_uint64, C.c_int32, C.c_double
+P = C.c_void_p
+PU32, PU64, PI32, PF64, PU8 = (C.POINTER(u32), C.POINTER(u64), C.POINTER(i32), C.POINTER(f64),
+                               C.POINTER(C.c_uint8))
+
+# status codes (BackendError variants)
+OK = 0
+ERR_DEVICE_FAILURE = 1
+ERR_QUBIT_OUT_OF_RANGE = 2
+ERR_DUPLICATE_QUBIT = 3
+ERR_ARITY_MISMATCH = 4
+ERR_UNSUPPORTED_GATE = 5
+ERR_UNSUPPORTED_INSTRUCTION = 6
+ERR_SYMBOLIC_PARAM = 7
+ERR_NON_FINITE_PARAM = 8
+ERR_NON_UNITARY_MATRIX = 9
+ERR_EMPTY_CIRCUIT = 10
+ERR_DEGENERATE_MEASUREMENT = 11
+ERR_TOO_MANY_QUBITS = 12
+ERR_INVALID_PAULI_STRING = 13
+ERR_INVALID_STATE = 14
+ERR_OPTIMIZATION = 15
+ERR_NO_DEVICE = 16
+ERR_INVALID_ARGUMENT = 17
+ERR_OUT_OF_MEMORY = 18
+
+F64, F32 = 64, 32
+PLANNER_UNFUSED, PLANNER_RULE, PLANNER_LEARNED = 0, 1, 2
+INSTR_DIAGONAL_PHASE, INSTR_BARRIER = 64, 65
+
+
+class Error(C.Structure):
+    _fields_ = [("code", i32), ("qubit", u32), ("num_qubits", u32), ("expected", u32), ("got", u32),
+                ("value", f64), ("kind", C.c_char * 48), ("message", C.c_char * 256)]
+
+
+class Gate(C.Structure):
+    _fields_ = [("kind", u32), ("n_qubits", u32), ("n_controls", u32), ("k", u32), ("qubits", PU32),
+                ("controls", PU32), ("params", f64 * 3), ("matrix", PF64)]
+
+
+class PlanOpts(C.Structure):
+    _fields_ = [("planner", u32), ("local_qubits", u32), ("max_ops_per_pass", u32), ("coalesce_bits", u32),
+                ("fuse_1q", u32), ("reserved", u32 * 3)]
+
+
+class PlanStats(C.Structure):
+    _fields_ = [("plan_ms", f64), ("input_gates", u32), ("n_passes", u32), ("n_stages", u32), ("n_ops", u32),
+                ("n_launches", u32), ("reserved", u32), ("sweep_bytes", f64)]
+
+
+# name -> (restype, argtypes); every symbol include/aleph_hip.h declares.
+SIGNATURES = {
+    "aleph_hip_device_count": (i32, [PI32]),
+    "aleph_hip_ctx_create": (i32, [i32, u64, C.POINTER(P)]),
+    "aleph_hip_ctx_destroy": (None, [P]),
+    "aleph_hip_ctx_set_qubit_cap": (i32, [P, u32]),
+    "aleph_hip_ctx_reseed": (i32, [P, u64]),
+    "aleph_hip_synchronize": (i32, [P]),
+    "aleph_hip_last_error": (i32, [P, C.POINTER(Error)]),
+    "aleph_hip_ctx_stream": (i32, [P, C.POINTER(P)]),
+    "aleph_hip_state_alloc": (i32, [P, u32, u32, C.POINTER(P)]),
+    "aleph_hip_state_free": (None, [P]),
+    "aleph_hip_state_reset": (i32, [P, P]),
+    "aleph_hip_state_load": (i32, [P, P, PF64]),
+    "aleph_hip_state_read": (i32, [P, P, PF64]),
+    "aleph_hip_state_load_device": (i32, [P, P, P]),
+    "aleph_hip_state_read_device": (i32, [P, P, P]),
+    "aleph_hip_state_info": (i32, [P, PU32, PU32, C.POINTER(P)]),
+    "aleph_hip_apply_gate": (i32, [P, P, C.POINTER(Gate)]),
+    "aleph_hip_apply_diagonal_phase": (i32, [P, P, u32, PF64, PU64, PU32]),
+    "aleph_hip_unpermute": (i32, [P, P, PU32]),
+    "aleph_hip_apply_1q": (i32, [P, P, u32, u64, PF64]),
+    "aleph_hip_apply_kq": (i32, [P, P, u32, PU32, u64, PF64]),
+    "aleph_hip_apply_diag": (i32, [P, P, u32, PU32, u64, PF64]),
+    "aleph_hip_apply_cnot": (i32, [P, P, u32, u32]),
+    "aleph_hip_apply_swap": (i32, [P, P, u32, u32]),
+    "aleph_hip_plan_create": (i32, [P, u32, u32, PU32, C.c_size_t, PF64, C.c_size_t, C.POINTER(PlanOpts),
+                                    C.POINTER(P)]),
+    "aleph_hip_plan_execute": (i32, [P, P, P]),
+    "aleph_hip_plan_execute_timed": (i32, [P, P, P, PF64, PU32]),
+    "aleph_hip_plan_stats_get": (i32, [P, C.POINTER(PlanStats)]),
+    "aleph_hip_plan_free": (None, [P]),
+    "aleph_hip_run_program": (i32, [P, P, PU32, C.c_size_t, PF64, C.c_size_t, C.POINTER(PlanOpts),
+                                    C.POINTER(PlanStats)]),
+    "aleph_hip_norm_sqr": (i32, [P, P, PF64]),
+    "aleph_hip_validate": (i32, [P, P]),
+    "aleph_hip_measure": (i32, [P, P, u32, PI32]),
+    "aleph_hip_measure_with_uniform": (i32, [P, P, u32, f64, PI32, PI32]),
+    "aleph_hip_sample": (i32, [P, P, u32, PU64]),
+    "aleph_hip_sample_with_uniforms": (i32, [P, P, PF64, u32, PU64]),
+    "aleph_hip_expectation": (i32, [P, P, f64, PU32, PU8, u32, PF64]),
+    "aleph_hip_probabilities": (i32, [P, P, PU32, u32, PF64]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the built library (raises if it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m aleph_amd.build` (or __graft_entry__.build())")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def device_count() -> int:
+    lib = load()
+    c = i32(0)
+    lib.aleph_hip_device_count(C.byref(c))
+    return int(c.value)

@@ -1,0 +1,744 @@
+// C ABI of the gfx950 state-vector backend (include/aleph_hip.h).
+//
+// Every entry point validates on the host before any launch (as
+// crates/aleph-cuda/src/common.rs `validate_and_extract` does), converts
+// failures into a status code plus structured detail kept on the context,
+// and orders all device work on the context's single stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "engine.hpp"
+#include "gates.hpp"
+#include "kernels.hpp"
+
+using namespace aleph;
+
+struct aleph_hip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t rng[4] = {0, 0, 0, 0};
+  uint32_t cap = 0;                 // 0: 32 qubits for F64, 33 for F32
+  aleph_hip_error err{};
+  double* d_scratch = nullptr;      // reductions + small matrices
+  size_t scratch_bytes = 0;
+  double* h_pinned = nullptr;       // 64 doubles of pinned readback
+};
+
+struct aleph_hip_state {
+  aleph_hip_ctx* ctx = nullptr;
+  uint32_t n = 0;
+  uint32_t precision = ALEPH_HIP_F64;
+  void* d = nullptr;
+};
+
+struct aleph_hip_plan {
+  std::unique_ptr<Plan> plan;
+};
+
+namespace {
+
+constexpr size_t kScratchBytes = 1u << 20;
+
+// xoshiro256** seeded through SplitMix64.
+void rng_seed(aleph_hip_ctx* c, uint64_t seed) {
+  for (int i = 0; i < 4; i++) {
+    seed += 0x9E3779B97F4A7C15ull;
+    uint64_t z = seed;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    c->rng[i] = z ^ (z >> 31);
+  }
+}
+
+uint64_t rng_next(aleph_hip_ctx* c) {
+  uint64_t* s = c->rng;
+  auto rotl = [](uint64_t x, int k) { return (x << k) | (x >> (64 - k)); };
+  const uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+  s[2] ^= s[0];
+  s[3] ^= s[1];
+  s[1] ^= s[2];
+  s[0] ^= s[3];
+  s[2] ^= t;
+  s[3] = rotl(s[3], 45);
+  return r;
+}
+
+double rng_uniform(aleph_hip_ctx* c) { return (double)(rng_next(c) >> 11) * (1.0 / 9007199254740992.0); }
+
+void set_error(aleph_hip_ctx* c, const Failure& f) {
+  if (!c) return;
+  c->err = aleph_hip_error{};
+  c->err.code = f.code;
+  c->err.qubit = f.qubit;
+  c->err.num_qubits = f.num_qubits;
+  c->err.expected = f.expected;
+  c->err.got = f.got;
+  c->err.value = f.value;
+  std::snprintf(c->err.kind, sizeof(c->err.kind), "%s", f.kind.c_str());
+  std::snprintf(c->err.message, sizeof(c->err.message), "%s", f.message.c_str());
+}
+
+template <typename F>
+aleph_hip_status guarded(aleph_hip_ctx* c, F&& body) {
+  try {
+    if (c) {
+      c->err = aleph_hip_error{};
+      ALEPH_HIP_CHECK(hipSetDevice(c->device));
+    }
+    body();
+    return ALEPH_HIP_OK;
+  } catch (const Failure& f) {
+    set_error(c, f);
+    return f.code;
+  } catch (const std::bad_alloc&) {
+    set_error(c, Failure{ALEPH_HIP_ERR_OUT_OF_MEMORY, "host allocation failed"});
+    return ALEPH_HIP_ERR_OUT_OF_MEMORY;
+  }
+}
+
+void need(bool cond, const char* what) {
+  if (!cond) {
+    Failure f{ALEPH_HIP_ERR_INVALID_ARGUMENT, what};
+    f.kind = "argument";
+    throw f;
+  }
+}
+
+void check_state(aleph_hip_ctx* c, aleph_hip_state* st) {
+  need(c != nullptr && st != nullptr, "null context or state");
+  need(st->ctx == c, "state belongs to another context");
+}
+
+void check_qubit(const aleph_hip_state* st, uint32_t q, const char* kind) {
+  if (q >= st->n) {
+    Failure f = make_failure(ALEPH_HIP_ERR_QUBIT_OUT_OF_RANGE, kind, "qubit out of range");
+    f.qubit = q;
+    f.num_qubits = st->n;
+    throw f;
+  }
+}
+
+void check_distinct(const uint32_t* qs, uint32_t k, const aleph_hip_state* st, const char* kind) {
+  uint64_t seen = 0;
+  for (uint32_t i = 0; i < k; i++) {
+    check_qubit(st, qs[i], kind);
+    if (seen & (1ull << qs[i])) {
+      Failure f = make_failure(ALEPH_HIP_ERR_DUPLICATE_QUBIT, kind, "duplicate qubit");
+      f.qubit = qs[i];
+      throw f;
+    }
+    seen |= 1ull << qs[i];
+  }
+}
+
+size_t amp_bytes(const aleph_hip_state* st) { return st->precision == ALEPH_HIP_F64 ? 16 : 8; }
+
+// Σ|ψ|² over bit q clear / set, non-finite count (q < 0: everything in [0]).
+void reduce(aleph_hip_ctx* c, aleph_hip_state* st, int32_t q, double out[3]) {
+  double* partials = c->d_scratch;
+  double* res = c->d_scratch + 3 * kReduceBlocks;
+  if (st->precision == ALEPH_HIP_F64) launch_reduce_branch<double>(c->stream, st->d, st->n, q, partials, res);
+  else launch_reduce_branch<float>(c->stream, st->d, st->n, q, partials, res);
+  ALEPH_HIP_CHECK(hipMemcpyAsync(c->h_pinned, res, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 3; i++) out[i] = c->h_pinned[i];
+}
+
+// validate_state (crates/aleph-sv/src/measure.rs:23-39).
+void validate(aleph_hip_ctx* c, aleph_hip_state* st, double* p0, double* p1, int32_t q) {
+  double r[3];
+  reduce(c, st, q, r);
+  const double total = r[0] + r[1];
+  const double tol = std::sqrt(std::ldexp(1.0, (int)st->n)) * kAmplitudeTol;
+  if (r[2] != 0.0) throw make_failure(ALEPH_HIP_ERR_INVALID_STATE, "non-finite amplitude norm²", "invalid state");
+  if (!std::isfinite(total) || std::fabs(total - 1.0) > tol)
+    throw make_failure(ALEPH_HIP_ERR_INVALID_STATE, "state norm² deviates from 1 beyond drift budget",
+                       "invalid state");
+  if (p0) *p0 = r[0];
+  if (p1) *p1 = r[1];
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  explicit DevBuf(size_t bytes) { ALEPH_HIP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16))); }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+aleph_hip_status measure_impl(aleph_hip_ctx* c, aleph_hip_state* st, uint32_t qubit, const double* forced_u,
+                              int32_t* outcome, int32_t* consumed) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(outcome != nullptr, "null outcome");
+    check_qubit(st, qubit, "measure");
+    double p0, p1;
+    validate(c, st, &p0, &p1, (int32_t)qubit);
+    p0 = std::min(std::max(p0, 0.0), 1.0);
+    p1 = std::min(std::max(p1, 0.0), 1.0);
+    const bool one_deg = p1 < 1e-300, zero_deg = p0 < 1e-300;  // measure.rs:12, 77-90
+    bool out;
+    double p;
+    if (consumed) *consumed = 0;
+    if (zero_deg && one_deg) {
+      Failure f = make_failure(ALEPH_HIP_ERR_DEGENERATE_MEASUREMENT, "measure", "degenerate branch");
+      f.qubit = qubit;
+      f.value = std::max(p0, p1);
+      throw f;
+    } else if (zero_deg) {
+      out = true;
+      p = p1;
+    } else if (one_deg) {
+      out = false;
+      p = p0;
+    } else {
+      const double u = forced_u ? *forced_u : rng_uniform(c);
+      if (consumed) *consumed = 1;
+      out = u < p1;
+      p = out ? p1 : p0;
+    }
+    const double scale = 1.0 / std::sqrt(p);
+    if (st->precision == ALEPH_HIP_F64) launch_collapse<double>(c->stream, st->d, st->n, qubit, out, scale);
+    else launch_collapse<float>(c->stream, st->d, st->n, qubit, out, scale);
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    *outcome = out ? 1 : 0;
+  });
+}
+
+aleph_hip_status sample_impl(aleph_hip_ctx* c, aleph_hip_state* st, const double* uniforms, uint32_t shots,
+                             uint64_t* out) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(shots == 0 || out != nullptr, "null output");
+    validate(c, st, nullptr, nullptr, -1);
+    if (shots == 0) return;
+    std::vector<double> u(shots);
+    for (uint32_t i = 0; i < shots; i++) u[i] = uniforms ? uniforms[i] : rng_uniform(c);
+    const uint64_t nb = st->n > kSampleBlockBits ? (1ull << (st->n - kSampleBlockBits)) : 1ull;
+    DevBuf blocks(2 * nb * sizeof(double));
+    DevBuf misc(64 * sizeof(double));
+    DevBuf du(shots * sizeof(double));
+    DevBuf dout(shots * sizeof(uint64_t));
+    ALEPH_HIP_CHECK(hipMemcpyAsync(du.p, u.data(), shots * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (st->precision == ALEPH_HIP_F64)
+      launch_sample<double>(c->stream, st->d, st->n, (double*)blocks.p, (double*)misc.p, (const double*)du.p, shots,
+                            (uint64_t*)dout.p);
+    else
+      launch_sample<float>(c->stream, st->d, st->n, (double*)blocks.p, (double*)misc.p, (const double*)du.p, shots,
+                           (uint64_t*)dout.p);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(out, dout.p, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+aleph_hip_status aleph_hip_device_count(int32_t* count) {
+  if (!count) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *count = c;
+  return ALEPH_HIP_OK;
+}
+
+aleph_hip_status aleph_hip_ctx_create(int32_t ordinal, uint64_t seed, aleph_hip_ctx** out) {
+  if (!out) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return ALEPH_HIP_ERR_NO_DEVICE;
+  if (ordinal < 0 || ordinal >= count) return ALEPH_HIP_ERR_NO_DEVICE;
+  auto* c = new aleph_hip_ctx;
+  c->device = ordinal;
+  rng_seed(c, seed);
+  const aleph_hip_status st = guarded(c, [&] {
+    ALEPH_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    ALEPH_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&c->d_scratch), kScratchBytes));
+    c->scratch_bytes = kScratchBytes;
+    ALEPH_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_pinned), 64 * sizeof(double), hipHostMallocDefault));
+  });
+  if (st != ALEPH_HIP_OK) {
+    aleph_hip_ctx_destroy(c);
+    return st;
+  }
+  *out = c;
+  return ALEPH_HIP_OK;
+}
+
+void aleph_hip_ctx_destroy(aleph_hip_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->d_scratch) (void)hipFree(c->d_scratch);
+  if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+aleph_hip_status aleph_hip_ctx_set_qubit_cap(aleph_hip_ctx* c, uint32_t cap) {
+  if (!c) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  c->cap = cap;
+  return ALEPH_HIP_OK;
+}
+
+aleph_hip_status aleph_hip_ctx_reseed(aleph_hip_ctx* c, uint64_t seed) {
+  if (!c) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  rng_seed(c, seed);
+  return ALEPH_HIP_OK;
+}
+
+aleph_hip_status aleph_hip_synchronize(aleph_hip_ctx* c) {
+  return guarded(c, [&] {
+    need(c != nullptr, "null context");
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+aleph_hip_status aleph_hip_last_error(const aleph_hip_ctx* c, aleph_hip_error* out) {
+  if (!c || !out) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  *out = c->err;
+  return ALEPH_HIP_OK;
+}
+
+aleph_hip_status aleph_hip_ctx_stream(aleph_hip_ctx* c, void** stream) {
+  if (!c || !stream) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  *stream = c->stream;
+  return ALEPH_HIP_OK;
+}
+
+aleph_hip_status aleph_hip_state_alloc(aleph_hip_ctx* c, uint32_t n, uint32_t precision, aleph_hip_state** out) {
+  return guarded(c, [&] {
+    need(c != nullptr && out != nullptr, "null argument");
+    *out = nullptr;
+    need(precision == ALEPH_HIP_F64 || precision == ALEPH_HIP_F32, "precision must be 64 or 32");
+    const uint32_t cap = c->cap ? c->cap : (precision == ALEPH_HIP_F64 ? 32u : 33u);
+    if (n > cap || n > 40) {
+      Failure f = make_failure(ALEPH_HIP_ERR_TOO_MANY_QUBITS, "allocate", "too many qubits");
+      f.expected = std::min<uint32_t>(cap, 40);
+      f.got = n;
+      throw f;
+    }
+    const size_t bytes = (precision == ALEPH_HIP_F64 ? 16ull : 8ull) << n;
+    size_t free_b = 0, total_b = 0;
+    ALEPH_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    if (bytes > free_b) {
+      Failure f{ALEPH_HIP_ERR_OUT_OF_MEMORY, "state does not fit in free device memory"};
+      f.kind = "allocate";
+      throw f;
+    }
+    auto st = std::make_unique<aleph_hip_state>();
+    st->ctx = c;
+    st->n = n;
+    st->precision = precision;
+    ALEPH_HIP_CHECK(hipMalloc(&st->d, bytes));
+    if (precision == ALEPH_HIP_F64) launch_init_basis<double>(c->stream, st->d, n);
+    else launch_init_basis<float>(c->stream, st->d, n);
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    *out = st.release();
+  });
+}
+
+void aleph_hip_state_free(aleph_hip_state* st) {
+  if (!st) return;
+  if (st->ctx) {
+    (void)hipSetDevice(st->ctx->device);
+    (void)hipStreamSynchronize(st->ctx->stream);
+  }
+  if (st->d) (void)hipFree(st->d);
+  delete st;
+}
+
+aleph_hip_status aleph_hip_state_reset(aleph_hip_ctx* c, aleph_hip_state* st) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    if (st->precision == ALEPH_HIP_F64) launch_init_basis<double>(c->stream, st->d, st->n);
+    else launch_init_basis<float>(c->stream, st->d, st->n);
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+aleph_hip_status aleph_hip_state_load(aleph_hip_ctx* c, aleph_hip_state* st, const double* amps) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(amps != nullptr, "null amplitudes");
+    const size_t len = 1ull << st->n;
+    if (st->precision == ALEPH_HIP_F64) {
+      ALEPH_HIP_CHECK(hipMemcpy(st->d, amps, len * 16, hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> f(2 * len);
+      for (size_t i = 0; i < 2 * len; i++) f[i] = (float)amps[i];
+      ALEPH_HIP_CHECK(hipMemcpy(st->d, f.data(), len * 8, hipMemcpyHostToDevice));
+    }
+  });
+}
+
+aleph_hip_status aleph_hip_state_read(aleph_hip_ctx* c, aleph_hip_state* st, double* out) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(out != nullptr, "null output");
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    const size_t len = 1ull << st->n;
+    if (st->precision == ALEPH_HIP_F64) {
+      ALEPH_HIP_CHECK(hipMemcpy(out, st->d, len * 16, hipMemcpyDeviceToHost));
+    } else {
+      std::vector<float> f(2 * len);
+      ALEPH_HIP_CHECK(hipMemcpy(f.data(), st->d, len * 8, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < 2 * len; i++) out[i] = f[i];
+    }
+  });
+}
+
+aleph_hip_status aleph_hip_state_load_device(aleph_hip_ctx* c, aleph_hip_state* st, const void* dptr) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(dptr != nullptr, "null pointer");
+    ALEPH_HIP_CHECK(hipMemcpyAsync(st->d, dptr, amp_bytes(st) << st->n, hipMemcpyDeviceToDevice, c->stream));
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+aleph_hip_status aleph_hip_state_read_device(aleph_hip_ctx* c, aleph_hip_state* st, void* dptr) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(dptr != nullptr, "null pointer");
+    ALEPH_HIP_CHECK(hipMemcpyAsync(dptr, st->d, amp_bytes(st) << st->n, hipMemcpyDeviceToDevice, c->stream));
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+aleph_hip_status aleph_hip_state_info(const aleph_hip_state* st, uint32_t* n, uint32_t* precision, void** dptr) {
+  if (!st) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  if (n) *n = st->n;
+  if (precision) *precision = st->precision;
+  if (dptr) *dptr = st->d;
+  return ALEPH_HIP_OK;
+}
+
+aleph_hip_status aleph_hip_apply_gate(aleph_hip_ctx* c, aleph_hip_state* st, const aleph_hip_gate* g) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(g != nullptr, "null gate");
+    need(g->n_qubits == 0 || g->qubits != nullptr, "null qubits");
+    need(g->n_controls == 0 || g->controls != nullptr, "null controls");
+    const DenseGate d = validate_gate(*g, st->n);
+    apply_dense_gate(c->stream, st->d, st->n, st->precision, d, c->d_scratch);
+  });
+}
+
+aleph_hip_status aleph_hip_apply_diagonal_phase(aleph_hip_ctx* c, aleph_hip_state* st, uint32_t n_terms,
+                                                const double* angles, const uint64_t* conds, const uint32_t* offsets) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(n_terms == 0 || (angles && offsets), "null term arrays");
+    if (n_terms == 0) return;
+    const uint32_t nc = offsets[n_terms];
+    need(offsets[0] == 0 && (nc == 0 || conds != nullptr), "bad CSR offsets");
+    for (uint32_t t = 0; t < n_terms; t++) {
+      need(offsets[t] <= offsets[t + 1], "non-monotone CSR offsets");
+      if (!std::isfinite(angles[t])) throw make_failure(ALEPH_HIP_ERR_NON_FINITE_PARAM, "DiagonalPhase", "angle");
+    }
+    for (uint32_t i = 0; i < nc; i++)
+      if (st->n < 64 && (conds[i] >> st->n) != 0) {
+        Failure f = make_failure(ALEPH_HIP_ERR_QUBIT_OUT_OF_RANGE, "DiagonalPhase", "mask beyond state");
+        f.qubit = 63 - __builtin_clzll(conds[i]);
+        f.num_qubits = st->n;
+        throw f;
+      }
+    DevBuf da(n_terms * sizeof(double)), dc((size_t)nc * sizeof(uint64_t)), dof((n_terms + 1) * sizeof(uint32_t));
+    ALEPH_HIP_CHECK(hipMemcpyAsync(da.p, angles, n_terms * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (nc) ALEPH_HIP_CHECK(hipMemcpyAsync(dc.p, conds, nc * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+    ALEPH_HIP_CHECK(hipMemcpyAsync(dof.p, offsets, (n_terms + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    if (st->precision == ALEPH_HIP_F64)
+      launch_phase_poly<double>(c->stream, st->d, st->n, n_terms, (const double*)da.p, (const uint64_t*)dc.p,
+                                (const uint32_t*)dof.p);
+    else
+      launch_phase_poly<float>(c->stream, st->d, st->n, n_terms, (const double*)da.p, (const uint64_t*)dc.p,
+                               (const uint32_t*)dof.p);
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+aleph_hip_status aleph_hip_unpermute(aleph_hip_ctx* c, aleph_hip_state* st, const uint32_t* perm) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(perm != nullptr, "null permutation");
+    uint64_t seen = 0;
+    for (uint32_t l = 0; l < st->n; l++) {
+      need(perm[l] < st->n && !(seen & (1ull << perm[l])), "not a permutation of 0..n-1");
+      seen |= 1ull << perm[l];
+    }
+    void* dst = nullptr;
+    ALEPH_HIP_CHECK(hipMalloc(&dst, amp_bytes(st) << st->n));
+    if (st->precision == ALEPH_HIP_F64) launch_permute<double>(c->stream, st->d, dst, st->n, perm);
+    else launch_permute<float>(c->stream, st->d, dst, st->n, perm);
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    (void)hipFree(st->d);
+    st->d = dst;
+  });
+}
+
+aleph_hip_status aleph_hip_apply_1q(aleph_hip_ctx* c, aleph_hip_state* st, uint32_t target, uint64_t cm,
+                                    const double* m) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(m != nullptr, "null matrix");
+    check_qubit(st, target, "apply_1q");
+    need(st->n == 64 || (cm >> st->n) == 0, "control mask beyond state");
+    need(!(cm & (1ull << target)), "control overlaps target");
+    if (st->precision == ALEPH_HIP_F64) launch_1q<double>(c->stream, st->d, st->n, target, cm, m);
+    else launch_1q<float>(c->stream, st->d, st->n, target, cm, m);
+  });
+}
+
+aleph_hip_status aleph_hip_apply_kq(aleph_hip_ctx* c, aleph_hip_state* st, uint32_t k, const uint32_t* qubits,
+                                    uint64_t cm, const double* m) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(qubits != nullptr && m != nullptr, "null argument");
+    need(k >= 1 && k <= kMaxFusedK && k <= st->n, "k must be in [1, 6] and <= n");
+    check_distinct(qubits, k, st, "apply_kq");
+    need(st->n == 64 || (cm >> st->n) == 0, "control mask beyond state");
+    for (uint32_t j = 0; j < k; j++) need(!(cm & (1ull << qubits[j])), "control overlaps target");
+    const size_t dim = 1ull << k;
+    if (st->precision == ALEPH_HIP_F64) {
+      ALEPH_HIP_CHECK(hipMemcpyAsync(c->d_scratch, m, 2 * dim * dim * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      launch_kq<double>(c->stream, st->d, st->n, k, qubits, cm, c->d_scratch);
+    } else {
+      std::vector<float> f(m, m + 2 * dim * dim);
+      ALEPH_HIP_CHECK(hipMemcpyAsync(c->d_scratch, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      launch_kq<float>(c->stream, st->d, st->n, k, qubits, cm, c->d_scratch);
+    }
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+aleph_hip_status aleph_hip_apply_diag(aleph_hip_ctx* c, aleph_hip_state* st, uint32_t k, const uint32_t* qubits,
+                                      uint64_t cm, const double* d) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(qubits != nullptr && d != nullptr, "null argument");
+    need(k >= 1 && k <= 16 && k <= st->n, "k must be in [1, 16] and <= n");
+    check_distinct(qubits, k, st, "apply_diag");
+    need(st->n == 64 || (cm >> st->n) == 0, "control mask beyond state");
+    const size_t bytes = (2ull << k) * sizeof(double);
+    need(bytes <= kScratchBytes, "diagonal too large");
+    ALEPH_HIP_CHECK(hipMemcpyAsync(c->d_scratch, d, bytes, hipMemcpyHostToDevice, c->stream));
+    if (st->precision == ALEPH_HIP_F64) launch_diag<double>(c->stream, st->d, st->n, k, qubits, cm, c->d_scratch);
+    else launch_diag<float>(c->stream, st->d, st->n, k, qubits, cm, c->d_scratch);
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+aleph_hip_status aleph_hip_apply_cnot(aleph_hip_ctx* c, aleph_hip_state* st, uint32_t control, uint32_t target) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    const uint32_t qs[2] = {control, target};
+    check_distinct(qs, 2, st, "apply_cnot");
+    const double x[8] = {0, 0, 1, 0, 1, 0, 0, 0};
+    if (st->precision == ALEPH_HIP_F64) launch_1q<double>(c->stream, st->d, st->n, target, 1ull << control, x);
+    else launch_1q<float>(c->stream, st->d, st->n, target, 1ull << control, x);
+  });
+}
+
+aleph_hip_status aleph_hip_apply_swap(aleph_hip_ctx* c, aleph_hip_state* st, uint32_t a, uint32_t b) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    const uint32_t qs[2] = {a, b};
+    check_distinct(qs, 2, st, "apply_swap");
+    if (st->precision == ALEPH_HIP_F64) launch_swap<double>(c->stream, st->d, st->n, a, b, 0);
+    else launch_swap<float>(c->stream, st->d, st->n, a, b, 0);
+  });
+}
+
+aleph_hip_status aleph_hip_plan_create(aleph_hip_ctx* c, uint32_t n, uint32_t precision, const uint32_t* words,
+                                       size_t n_words, const double* reals, size_t n_reals,
+                                       const aleph_hip_plan_opts* opts, aleph_hip_plan** out) {
+  return guarded(c, [&] {
+    need(c != nullptr && out != nullptr, "null argument");
+    *out = nullptr;
+    need(precision == ALEPH_HIP_F64 || precision == ALEPH_HIP_F32, "precision must be 64 or 32");
+    need(n >= 1 && n <= 40, "qubit count must be in [1, 40]");
+    need(n_words == 0 || words != nullptr, "null words");
+    need(n_reals == 0 || reals != nullptr, "null reals");
+    PlanOptions po;
+    if (opts) {
+      po.planner = opts->planner;
+      po.local_qubits = opts->local_qubits;
+      po.max_ops = opts->max_ops_per_pass;
+      po.coalesce_bits = opts->coalesce_bits;
+      po.fuse_1q = opts->fuse_1q != 0;
+    }
+    need(po.planner <= ALEPH_HIP_PLANNER_LEARNED, "unknown planner");
+    need(po.local_qubits == 0 || (po.local_qubits >= (uint32_t)kRegBits && po.local_qubits <= (uint32_t)kMaxLocalBits),
+         "local_qubits must be in [4, 14]");
+    need(po.local_qubits == 0 || precision == ALEPH_HIP_F32 || po.local_qubits <= 13, "F64 chunks are at most 2^13");
+    const std::vector<ProgInstr> prog = decode_program(words, n_words, reals, n_reals, n);
+    auto h = std::make_unique<aleph_hip_plan>();
+    h->plan = build_plan(prog, n, precision, po);
+    h->plan->upload(c->stream);
+    *out = h.release();
+  });
+}
+
+aleph_hip_status aleph_hip_plan_execute(aleph_hip_ctx* c, aleph_hip_plan* plan, aleph_hip_state* st) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(plan != nullptr && plan->plan != nullptr, "null plan");
+    need(plan->plan->n == st->n && plan->plan->precision == st->precision, "plan built for another state shape");
+    plan->plan->execute(c->stream, st->d);
+  });
+}
+
+aleph_hip_status aleph_hip_plan_execute_timed(aleph_hip_ctx* c, aleph_hip_plan* plan, aleph_hip_state* st,
+                                             double* ms, uint32_t* kinds) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(plan != nullptr && plan->plan != nullptr, "null plan");
+    need(ms != nullptr, "null timing output");
+    const Plan& p = *plan->plan;
+    need(p.n == st->n && p.precision == st->precision, "plan built for another state shape");
+    const size_t ns = p.steps.size();
+    std::vector<hipEvent_t> ev(ns + 1, nullptr);
+    struct Guard {
+      std::vector<hipEvent_t>& e;
+      ~Guard() {
+        for (hipEvent_t x : e)
+          if (x) (void)hipEventDestroy(x);
+      }
+    } guard{ev};
+    for (auto& e : ev) ALEPH_HIP_CHECK(hipEventCreate(&e));
+    p.execute_with_events(c->stream, st->d, ev.data());
+    ALEPH_HIP_CHECK(hipEventSynchronize(ev[ns]));
+    for (size_t i = 0; i < ns; i++) {
+      float t = 0.f;
+      ALEPH_HIP_CHECK(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+      ms[i] = t;
+      if (kinds) kinds[i] = (uint32_t)p.steps[i].kind;
+    }
+  });
+}
+
+aleph_hip_status aleph_hip_plan_stats_get(const aleph_hip_plan* plan, aleph_hip_plan_stats* out) {
+  if (!plan || !plan->plan || !out) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  *out = plan->plan->stats;
+  return ALEPH_HIP_OK;
+}
+
+void aleph_hip_plan_free(aleph_hip_plan* plan) { delete plan; }
+
+aleph_hip_status aleph_hip_run_program(aleph_hip_ctx* c, aleph_hip_state* st, const uint32_t* words, size_t n_words,
+                                       const double* reals, size_t n_reals, const aleph_hip_plan_opts* opts,
+                                       aleph_hip_plan_stats* stats) {
+  if (!c || !st) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  aleph_hip_plan* plan = nullptr;
+  aleph_hip_status s = aleph_hip_plan_create(c, st->n, st->precision, words, n_words, reals, n_reals, opts, &plan);
+  if (s != ALEPH_HIP_OK) return s;
+  s = aleph_hip_plan_execute(c, plan, st);
+  if (s == ALEPH_HIP_OK) s = aleph_hip_synchronize(c);
+  if (s == ALEPH_HIP_OK && stats) aleph_hip_plan_stats_get(plan, stats);
+  aleph_hip_plan_free(plan);
+  return s;
+}
+
+aleph_hip_status aleph_hip_norm_sqr(aleph_hip_ctx* c, aleph_hip_state* st, double* out) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(out != nullptr, "null output");
+    double r[3];
+    reduce(c, st, -1, r);
+    *out = r[2] != 0.0 ? NAN : r[0] + r[1];
+  });
+}
+
+aleph_hip_status aleph_hip_validate(aleph_hip_ctx* c, aleph_hip_state* st) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    validate(c, st, nullptr, nullptr, -1);
+  });
+}
+
+aleph_hip_status aleph_hip_measure(aleph_hip_ctx* c, aleph_hip_state* st, uint32_t qubit, int32_t* outcome) {
+  return measure_impl(c, st, qubit, nullptr, outcome, nullptr);
+}
+
+aleph_hip_status aleph_hip_measure_with_uniform(aleph_hip_ctx* c, aleph_hip_state* st, uint32_t qubit, double u,
+                                                int32_t* outcome, int32_t* consumed) {
+  return measure_impl(c, st, qubit, &u, outcome, consumed);
+}
+
+aleph_hip_status aleph_hip_sample(aleph_hip_ctx* c, aleph_hip_state* st, uint32_t shots, uint64_t* out) {
+  return sample_impl(c, st, nullptr, shots, out);
+}
+
+aleph_hip_status aleph_hip_sample_with_uniforms(aleph_hip_ctx* c, aleph_hip_state* st, const double* u, uint32_t shots,
+                                                uint64_t* out) {
+  if (shots && !u) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  return sample_impl(c, st, u, shots, out);
+}
+
+aleph_hip_status aleph_hip_expectation(aleph_hip_ctx* c, aleph_hip_state* st, double coeff, const uint32_t* qubits,
+                                       const uint8_t* paulis, uint32_t nt, double* out) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(out != nullptr && (nt == 0 || (qubits && paulis)), "null argument");
+    validate(c, st, nullptr, nullptr, -1);
+    if (!std::isfinite(coeff))
+      throw make_failure(ALEPH_HIP_ERR_INVALID_PAULI_STRING, "non-finite coefficient", "invalid Pauli string");
+    check_distinct(qubits, nt, st, "expectation");
+    uint64_t x = 0, zy = 0;
+    int ny = 0;
+    for (uint32_t i = 0; i < nt; i++) {
+      need(paulis[i] <= 3, "pauli code must be 0..3");
+      const uint64_t b = 1ull << qubits[i];
+      if (paulis[i] == 1) x |= b;
+      if (paulis[i] == 2) { x |= b; zy |= b; ny++; }
+      if (paulis[i] == 3) zy |= b;
+    }
+    double* partials = c->d_scratch;
+    double* res = c->d_scratch + 3 * kReduceBlocks;
+    if (st->precision == ALEPH_HIP_F64) launch_expect_pauli<double>(c->stream, st->d, st->n, x, zy, partials, res);
+    else launch_expect_pauli<float>(c->stream, st->d, st->n, x, zy, partials, res);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(c->h_pinned, res, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    const double re = c->h_pinned[0], im = c->h_pinned[1];
+    double v;
+    switch (ny & 3) {  // Re(i^ny * (re + i im))
+      case 0: v = re; break;
+      case 1: v = -im; break;
+      case 2: v = -re; break;
+      default: v = im; break;
+    }
+    *out = coeff * v;
+  });
+}
+
+aleph_hip_status aleph_hip_probabilities(aleph_hip_ctx* c, aleph_hip_state* st, const uint32_t* qubits, uint32_t nq,
+                                         double* out) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(out != nullptr && (nq == 0 || qubits), "null argument");
+    validate(c, st, nullptr, nullptr, -1);
+    check_distinct(qubits, nq, st, "probabilities");
+    if (nq == 0) {
+      out[0] = 1.0;
+      return;
+    }
+    need(nq <= 30, "at most 30 marginal qubits");
+    const size_t bins = 1ull << nq;
+    DevBuf d(bins * sizeof(double));
+    if (st->precision == ALEPH_HIP_F64) launch_marginal<double>(c->stream, st->d, st->n, nq, qubits, (double*)d.p);
+    else launch_marginal<float>(c->stream, st->d, st->n, nq, qubits, (double*)d.p);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(out, d.p, bins * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+}  // extern "C"

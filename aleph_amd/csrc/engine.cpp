@@ -1,0 +1,850 @@
+// Program decoding, gate lowering, the rule-based pass planner and plan
+// execution for the gfx950 state-vector backend.
+//
+// The planner replaces the reference's per-gate execution
+// (crates/aleph-cuda/src/execution.rs:78-131 + sv/backend.rs:515-581, one
+// launch per gate or per <=3 disjoint 1q gates) with full-state passes that
+// each apply as many gates as a dependency-respecting greedy schedule fits
+// into one chunk of m qubits, under a per-pass arithmetic budget.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "kernels.hpp"
+
+namespace aleph {
+
+namespace {
+
+inline uint64_t qmask(const std::vector<uint32_t>& qs) {
+  uint64_t m = 0;
+  for (uint32_t q : qs) m |= 1ull << q;
+  return m;
+}
+
+inline int popc(uint64_t x) { return __builtin_popcountll(x); }
+
+Failure bad_program(const char* why) {
+  Failure f{ALEPH_HIP_ERR_INVALID_ARGUMENT, std::string("malformed program: ") + why};
+  f.kind = "program";
+  return f;
+}
+
+}  // namespace
+
+std::vector<ProgInstr> decode_program(const uint32_t* words, size_t n_words, const double* reals, size_t n_reals,
+                                      uint32_t n) {
+  std::vector<ProgInstr> out;
+  size_t w = 0, r = 0;
+  auto need_w = [&](size_t c) { if (w + c > n_words) throw bad_program("word stream truncated"); };
+  auto need_r = [&](size_t c) { if (r + c > n_reals) throw bad_program("real stream truncated"); };
+  while (w < n_words) {
+    const uint32_t tag = words[w];
+    if (tag == ALEPH_HIP_INSTR_BARRIER) {
+      need_w(2);
+      const uint32_t cnt = words[w + 1];
+      need_w(2 + (size_t)cnt);
+      for (uint32_t i = 0; i < cnt; i++)
+        if (words[w + 2 + i] >= n) {
+          Failure f = make_failure(ALEPH_HIP_ERR_QUBIT_OUT_OF_RANGE, "barrier", "qubit out of range");
+          f.qubit = words[w + 2 + i];
+          f.num_qubits = n;
+          throw f;
+        }
+      w += 2 + cnt;
+      ProgInstr pi;
+      pi.type = ProgInstr::BARRIER;
+      out.push_back(std::move(pi));
+      continue;
+    }
+    if (tag == ALEPH_HIP_INSTR_DIAGONAL_PHASE) {
+      need_w(2);
+      const uint32_t nt = words[w + 1];
+      w += 2;
+      ProgInstr pi;
+      pi.type = ProgInstr::DIAG_PHASE;
+      need_r(nt);
+      for (uint32_t t = 0; t < nt; t++) {
+        need_w(1);
+        const uint32_t nc = words[w++];
+        need_w(2 * (size_t)nc);
+        std::vector<uint64_t> cs;
+        for (uint32_t c = 0; c < nc; c++) {
+          const uint64_t v = (uint64_t)words[w] | ((uint64_t)words[w + 1] << 32);
+          w += 2;
+          if (n < 64 && (v >> n) != 0) {
+            Failure f = make_failure(ALEPH_HIP_ERR_QUBIT_OUT_OF_RANGE, "DiagonalPhase", "mask beyond state");
+            f.qubit = 63 - __builtin_clzll(v);
+            f.num_qubits = n;
+            throw f;
+          }
+          cs.push_back(v);
+        }
+        pi.conds.push_back(std::move(cs));
+        const double a = reals[r++];
+        if (!std::isfinite(a)) throw make_failure(ALEPH_HIP_ERR_NON_FINITE_PARAM, "DiagonalPhase", "non-finite angle");
+        pi.angles.push_back(a);
+      }
+      out.push_back(std::move(pi));
+      continue;
+    }
+    need_w(4);
+    aleph_hip_gate g{};
+    g.kind = tag;
+    g.n_qubits = words[w + 1];
+    g.n_controls = words[w + 2];
+    g.k = words[w + 3];
+    need_w(4 + (size_t)g.n_qubits + g.n_controls);
+    g.qubits = words + w + 4;
+    g.controls = words + w + 4 + g.n_qubits;
+    w += 4 + (size_t)g.n_qubits + g.n_controls;
+    if (g.kind >= ALEPH_HIP_GATE_COUNT) throw make_failure(ALEPH_HIP_ERR_UNSUPPORTED_GATE, "unknown", "unknown kind");
+    if (g.kind == ALEPH_HIP_GATE_UNITARYKQ && (g.k < 1 || g.k > 16)) throw bad_program("UnitaryKq width");
+    const int np = gate_param_count(g.kind);
+    need_r((size_t)np);
+    for (int i = 0; i < np; i++) g.params[i] = reals[r + i];
+    r += np;
+    const size_t pr = gate_payload_reals(g.kind, g.k);
+    need_r(pr);
+    g.matrix = pr ? reals + r : nullptr;
+    r += pr;
+    ProgInstr pi;
+    pi.type = ProgInstr::GATE;
+    pi.gate = validate_gate(g, n);
+    out.push_back(std::move(pi));
+  }
+  if (r != n_reals) throw bad_program("unused reals");
+  return out;
+}
+
+namespace {
+
+// Decompose a diagonal whose entries are unit-modulus into AND-mask phase
+// terms: alpha(x) = sum_S beta_S * prod_{q in S} x_q (Moebius inversion over
+// subsets of the target bits), each term also requiring every control.
+void diag_to_terms(const DenseGate& d, uint64_t cm, std::vector<std::pair<uint64_t, double>>& terms) {
+  const uint32_t k = (uint32_t)d.qubits.size();
+  const uint32_t dim = 1u << k;
+  std::vector<double> alpha(dim);
+  for (uint32_t j = 0; j < dim; j++) {
+    const cplx e = d.m[(size_t)j * dim + j];
+    alpha[j] = std::atan2(e.imag(), e.real());
+  }
+  for (uint32_t S = 0; S < dim; S++) {
+    double beta = 0.0;
+    for (uint32_t T = S;; T = (T - 1) & S) {
+      const int sign = (popc(S ^ T) & 1) ? -1 : 1;
+      beta += sign * alpha[T];
+      if (T == 0) break;
+    }
+    if (beta == 0.0) continue;
+    uint64_t mask = cm;
+    for (uint32_t p = 0; p < k; p++)
+      if ((S >> p) & 1u) mask |= 1ull << d.qubits[k - 1 - p];
+    terms.emplace_back(mask, beta);
+  }
+}
+
+LGate lower_gate(const DenseGate& d) {
+  LGate g;
+  const uint64_t cm = qmask(d.controls);
+  const uint32_t k = (uint32_t)d.qubits.size();
+  const uint32_t dim = d.dim;
+  g.cmask = cm;
+  if (d.diagonal) {
+    bool sign = true;
+    for (uint32_t j = 0; j < dim; j++) {
+      const cplx e = d.m[(size_t)j * dim + j];
+      const double want = j + 1 == dim ? -1.0 : 1.0;
+      if (e.real() != want || e.imag() != 0.0) sign = false;
+    }
+    g.dg = d.qubits;
+    g.dg.insert(g.dg.end(), d.controls.begin(), d.controls.end());
+    if (sign) {
+      g.kind = LGate::SIGN;
+      g.sign_mask = cm | qmask(d.qubits);
+      g.cost = 0.2;
+      return g;
+    }
+    bool unit = true;
+    for (uint32_t j = 0; j < dim; j++)
+      if (std::fabs(std::abs(d.m[(size_t)j * dim + j]) - 1.0) > 1e-14) unit = false;
+    if (unit && k <= 6) {
+      g.kind = LGate::TERMS;
+      diag_to_terms(d, cm, g.terms);
+      g.cost = 0.3 * (double)g.terms.size();
+      return g;
+    }
+    if (k <= 2) {
+      g.kind = LGate::DIAGX;
+      g.qa = d.qubits[0];
+      g.qb = k == 2 ? d.qubits[1] : 0;
+      g.n_diag_targets = k;
+      for (uint32_t sel = 0; sel < 4; sel++) {
+        const uint32_t j = k == 2 ? sel : (sel >> 1);
+        const cplx e = d.m[(size_t)j * dim + j];
+        g.m[2 * sel] = e.real();
+        g.m[2 * sel + 1] = e.imag();
+      }
+      g.cost = 1.0;
+      return g;
+    }
+    // wide non-unit diagonal: dense standalone sweep
+  } else {
+    // Controlled structure by kind: the control qubits act diagonally.
+    uint32_t tgt = UINT32_MAX;
+    uint64_t extra = 0;
+    std::vector<cplx> blk;
+    if (d.kind == ALEPH_HIP_GATE_CNOT) {
+      tgt = d.qubits[1];
+      extra = 1ull << d.qubits[0];
+      blk = {0, 1, 1, 0};
+    } else if (d.kind == ALEPH_HIP_GATE_TOFFOLI) {
+      tgt = d.qubits[2];
+      extra = (1ull << d.qubits[0]) | (1ull << d.qubits[1]);
+      blk = {0, 1, 1, 0};
+    } else if (d.kind == ALEPH_HIP_GATE_CRX || d.kind == ALEPH_HIP_GATE_CRY) {
+      tgt = d.qubits[1];
+      extra = 1ull << d.qubits[0];
+      blk = {d.m[10], d.m[11], d.m[14], d.m[15]};
+    }
+    if (tgt != UINT32_MAX) {
+      g.kind = LGate::G1Q;
+      g.qa = tgt;
+      g.cmask = cm | extra;
+      g.nd = {tgt};
+      for (uint32_t q = 0; q < 64; q++)
+        if ((g.cmask >> q) & 1ull) g.dg.push_back(q);
+      for (int e = 0; e < 4; e++) {
+        g.m[2 * e] = blk[e].real();
+        g.m[2 * e + 1] = blk[e].imag();
+      }
+      g.cost = 1.0;
+      return g;
+    }
+    if (dim == 2 || dim == 4) {
+      g.kind = dim == 2 ? LGate::G1Q : LGate::G2Q;
+      g.qa = d.qubits[0];
+      g.qb = dim == 4 ? d.qubits[1] : 0;
+      g.nd = d.qubits;
+      g.dg = d.controls;
+      for (uint32_t e = 0; e < dim * dim; e++) {
+        g.m[2 * e] = d.m[e].real();
+        g.m[2 * e + 1] = d.m[e].imag();
+      }
+      g.cost = dim == 2 ? 1.0 : 4.0;
+      return g;
+    }
+  }
+  g.kind = LGate::KQ;
+  g.k = k;
+  g.kq_qubits = d.qubits;
+  g.nd = d.qubits;
+  g.dg = d.controls;
+  g.kq_mat.resize(2 * (size_t)dim * dim);
+  for (size_t e = 0; e < (size_t)dim * dim; e++) {
+    g.kq_mat[2 * e] = d.m[e].real();
+    g.kq_mat[2 * e + 1] = d.m[e].imag();
+  }
+  g.cost = 1e9;
+  return g;
+}
+
+}  // namespace
+
+std::vector<LGate> lower_program(const std::vector<ProgInstr>& prog, uint32_t n) {
+  std::vector<LGate> out;
+  for (const ProgInstr& pi : prog) {
+    if (pi.type == ProgInstr::BARRIER) continue;
+    if (pi.type == ProgInstr::GATE) {
+      out.push_back(lower_gate(pi.gate));
+      continue;
+    }
+    // DiagonalPhase: single-bit parity conditions are AND masks.
+    LGate g;
+    bool simple = true;
+    for (const auto& cs : pi.conds)
+      for (uint64_t c : cs)
+        if (popc(c) != 1) simple = false;
+    uint64_t support = 0;
+    for (const auto& cs : pi.conds)
+      for (uint64_t c : cs) support |= c;
+    for (uint32_t q = 0; q < 64 && q < n; q++)
+      if ((support >> q) & 1ull) g.dg.push_back(q);
+    if (simple) {
+      g.kind = LGate::TERMS;
+      for (size_t t = 0; t < pi.angles.size(); t++) {
+        uint64_t m = 0;
+        for (uint64_t c : pi.conds[t]) m |= c;
+        g.terms.emplace_back(m, pi.angles[t]);
+      }
+      g.cost = 0.3 * (double)g.terms.size();
+    } else {
+      g.kind = LGate::POLY;
+      g.poly_off.push_back(0);
+      for (size_t t = 0; t < pi.angles.size(); t++) {
+        for (uint64_t c : pi.conds[t]) g.poly_conds.push_back(c);
+        g.poly_off.push_back((uint32_t)g.poly_conds.size());
+        g.poly_angles.push_back(pi.angles[t]);
+      }
+      g.cost = 1e9;
+    }
+    out.push_back(std::move(g));
+  }
+  return out;
+}
+
+namespace {
+
+// Merge runs of uncontrolled 1q gates on one qubit (the effect of the
+// reference's Fuse1qRuns, crates/aleph-ir/src/passes/fuse_1q.rs): M = G_k ... G_1.
+std::vector<LGate> fuse_1q_runs(const std::vector<LGate>& in) {
+  std::vector<LGate> out;
+  std::vector<int64_t> open(64, -1);  // index in `out` of the open 1q run per qubit
+  for (const LGate& g : in) {
+    if (g.kind == LGate::G1Q && g.cmask == 0) {
+      const uint32_t q = g.qa;
+      if (open[q] >= 0) {
+        LGate& acc = out[(size_t)open[q]];
+        double r[8];
+        const double* a = g.m;     // new gate (left factor)
+        const double* b = acc.m;   // accumulated product
+        for (int i = 0; i < 2; i++)
+          for (int j = 0; j < 2; j++) {
+            double re = 0, im = 0;
+            for (int c = 0; c < 2; c++) {
+              const double ar = a[2 * (i * 2 + c)], ai = a[2 * (i * 2 + c) + 1];
+              const double br = b[2 * (c * 2 + j)], bi = b[2 * (c * 2 + j) + 1];
+              re += ar * br - ai * bi;
+              im += ar * bi + ai * br;
+            }
+            r[2 * (i * 2 + j)] = re;
+            r[2 * (i * 2 + j) + 1] = im;
+          }
+        std::memcpy(acc.m, r, sizeof(r));
+        continue;
+      }
+      open[q] = (int64_t)out.size();
+      out.push_back(g);
+      continue;
+    }
+    for (uint32_t q : g.nd) open[q] = -1;
+    for (uint32_t q : g.dg) open[q] = -1;
+    out.push_back(g);
+  }
+  return out;
+}
+
+struct PlannerCtx {
+  uint32_t n, m, L;
+  double budget;
+};
+
+// Greedy dependency-respecting selection: a gate can join the current group
+// if none of its qubits is blocked by an earlier gate left out of the group.
+// Gates left out block their non-diagonal qubits for everything later and
+// their diagonal qubits for later non-diagonal use (diagonal actions on a
+// shared qubit commute).
+struct Blocker {
+  uint64_t any = 0, nd = 0;
+  bool admits(uint64_t ndm, uint64_t dgm) const { return !(ndm & (any | nd)) && !(dgm & any); }
+  void block(uint64_t ndm, uint64_t dgm) {
+    any |= ndm;
+    nd |= dgm;
+  }
+};
+
+uint32_t pair_class(uint32_t rm) {
+  switch (rm) {
+    case 0x3: return 5;
+    case 0x5: return 6;
+    case 0x9: return 7;
+    case 0x6: return 8;
+    case 0xA: return 9;
+    case 0xC: return 10;
+    default: return 11;
+  }
+}
+
+void swap_2q_matrix(double* m) {
+  // index (a b) -> (b a): swap bit 0 and bit 1 of rows and columns
+  double t[32];
+  auto sw = [](int x) { return ((x & 1) << 1) | ((x >> 1) & 1); };
+  for (int r = 0; r < 4; r++)
+    for (int c = 0; c < 4; c++) {
+      t[2 * (r * 4 + c)] = m[2 * (sw(r) * 4 + sw(c))];
+      t[2 * (r * 4 + c) + 1] = m[2 * (sw(r) * 4 + sw(c)) + 1];
+    }
+  std::memcpy(m, t, sizeof(t));
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+template <typename T>
+size_t push_matrix(std::vector<unsigned char>& payload, const std::vector<double>& mat) {
+  const size_t off = align256(payload.size());
+  payload.resize(off + mat.size() * sizeof(T));
+  T* dst = reinterpret_cast<T*>(payload.data() + off);
+  for (size_t i = 0; i < mat.size(); i++) dst[i] = (T)mat[i];
+  return off;
+}
+
+template <typename V>
+size_t push_raw(std::vector<unsigned char>& payload, const std::vector<V>& v) {
+  const size_t off = align256(payload.size());
+  payload.resize(off + v.size() * sizeof(V) + 8);
+  if (!v.empty()) std::memcpy(payload.data() + off, v.data(), v.size() * sizeof(V));
+  return off;
+}
+
+void add_standalone(Plan& p, const LGate& g) {
+  Step s;
+  if (g.kind == LGate::KQ) {
+    s.kind = Step::KQ;
+    s.k = g.k;
+    for (uint32_t j = 0; j < g.k; j++) s.qubits[j] = g.kq_qubits[j];
+    s.cmask = qmask(g.dg) & ~qmask(g.nd);
+    s.dev_off = p.precision == ALEPH_HIP_F64 ? push_matrix<double>(p.payload, g.kq_mat)
+                                             : push_matrix<float>(p.payload, g.kq_mat);
+  } else {
+    s.kind = Step::POLY;
+    s.count = (uint32_t)g.poly_angles.size();
+    s.dev_off = push_raw(p.payload, g.poly_angles);
+    s.dev_off2 = push_raw(p.payload, g.poly_conds);
+    s.dev_off3 = push_raw(p.payload, g.poly_off);
+  }
+  p.steps.push_back(s);
+}
+
+// Build one fused pass from the chosen gates.
+void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vector<const LGate*>& gates) {
+  // chunk qubits: sorted physical positions
+  std::vector<uint32_t> pos;
+  for (uint32_t q = 0; q < pc.n; q++)
+    if ((S >> q) & 1ull) pos.push_back(q);
+  uint32_t local_of[64];
+  for (uint32_t i = 0; i < 64; i++) local_of[i] = UINT32_MAX;
+  for (uint32_t i = 0; i < pos.size(); i++) local_of[pos[i]] = i;
+  auto local_mask = [&](uint64_t phys) {
+    uint32_t lm = 0;
+    for (uint32_t q = 0; q < 64; q++)
+      if ((phys >> q) & 1ull) lm |= 1u << local_of[q];
+    return lm;
+  };
+  const uint32_t m = (uint32_t)pos.size();
+  const uint32_t low_local = (1u << pc.L) - 1u;  // local bits kept as thread bits at HBM stages
+
+  // ---- stage formation ----
+  std::vector<std::pair<uint32_t, std::vector<const LGate*>>> stage_list;  // (G local mask, ops)
+  std::vector<const LGate*> rem = gates;
+  bool first = true;
+  while (!rem.empty()) {
+    auto form = [&](bool avoid_low) {
+      Blocker bl;
+      uint32_t G = 0;
+      std::vector<const LGate*> take, left;
+      for (const LGate* g : rem) {
+        const uint64_t ndm = qmask(g->nd), dgm = qmask(g->dg);
+        const uint32_t ndl = local_mask(ndm);
+        bool ok = bl.admits(ndm, dgm) && popc(G | ndl) <= kRegBits;
+        if (ok && avoid_low && (ndl & low_local)) ok = false;
+        if (ok) {
+          take.push_back(g);
+          G |= ndl;
+        } else {
+          bl.block(ndm, dgm);
+          left.push_back(g);
+        }
+      }
+      return std::make_tuple(G, take, left);
+    };
+    auto [G, take, left] = form(first && pc.L > 0);
+    bool has_dense = false;
+    for (const LGate* g : take) has_dense |= !g->nd.empty();
+    if (first && pc.L > 0 && !has_dense) std::tie(G, take, left) = form(false);
+    // pad the register set to kRegBits local bits, preferring high local bits
+    for (int b = (int)m - 1; b >= 0 && popc(G) < kRegBits; b--)
+      if (!(G & (1u << b))) G |= 1u << b;
+    stage_list.emplace_back(G, take);
+    rem = left;
+    first = false;
+  }
+  if (stage_list.empty()) stage_list.emplace_back(0u, std::vector<const LGate*>{});
+  // Re-layout stages so that HBM stages keep the low chunk bits as thread bits.
+  const uint32_t top = ((1u << m) - 1u) & ~((1u << (m - kRegBits)) - 1u);
+  if (pc.L > 0 && (stage_list.front().first & low_local))
+    stage_list.insert(stage_list.begin(), {top, {}});
+  if (pc.L > 0 && (stage_list.back().first & low_local)) stage_list.push_back({top, {}});
+
+  PassDev pd{};
+  pd.m = m;
+  pd.n_stages = (uint32_t)stage_list.size();
+  pd.stage_begin = (uint32_t)p.stages.size();
+  for (uint32_t i = 0; i < m; i++) pd.local_pos[i] = pos[i];
+
+  for (auto& [G, sgates] : stage_list) {
+    StageDev sd{};
+    uint32_t reg_local[kRegBits];
+    int rcount = 0;
+    for (uint32_t b = 0; b < m; b++)
+      if (G & (1u << b)) reg_local[rcount++] = b;
+    uint32_t reg_of_local[32];
+    for (int i = 0; i < 32; i++) reg_of_local[i] = UINT32_MAX;
+    uint64_t rphys = 0;
+    for (int g = 0; g < kRegBits; g++) {
+      reg_of_local[reg_local[g]] = (uint32_t)g;
+      sd.rl[g] = 1u << reg_local[g];
+      sd.rg[g] = 1ull << pos[reg_local[g]];
+      rphys |= sd.rg[g];
+    }
+    uint32_t j = 0;
+    for (uint32_t b = 0; b < m; b++) {
+      if (G & (1u << b)) continue;
+      sd.tl[j] = 1u << b;
+      sd.tg[j] = 1ull << pos[b];
+      j++;
+    }
+    sd.n_thread_bits = j;
+    sd.op_begin = (uint32_t)p.ops.size();
+    auto reg_mask = [&](uint64_t phys) {
+      uint32_t rm = 0;
+      for (int g = 0; g < kRegBits; g++)
+        if (phys & sd.rg[g]) rm |= 1u << g;
+      return rm;
+    };
+    std::vector<PhaseTerm> pend_reg, pend_gen;
+    auto flush_phase = [&]() {
+      if (pend_reg.empty() && pend_gen.empty()) return;
+      DevOp ph{};
+      ph.type = OP_PHASE;
+      ph.term_off = (uint32_t)p.terms.size();
+      ph.n_terms = (uint32_t)pend_reg.size();
+      ph.b = (uint32_t)pend_gen.size();
+      p.terms.insert(p.terms.end(), pend_reg.begin(), pend_reg.end());
+      p.terms.insert(p.terms.end(), pend_gen.begin(), pend_gen.end());
+      p.ops.push_back(ph);
+      pend_reg.clear();
+      pend_gen.clear();
+    };
+    for (const LGate* g : sgates) {
+      DevOp op{};
+      if (g->kind != LGate::TERMS) flush_phase();
+      switch (g->kind) {
+        case LGate::G1Q:
+          op.type = OP_1Q;
+          op.a = reg_of_local[local_of[g->qa]];
+          op.cmask = g->cmask;
+          std::memcpy(op.m, g->m, 8 * sizeof(double));
+          p.ops.push_back(op);
+          break;
+        case LGate::G2Q: {
+          op.type = OP_2Q;
+          uint32_t ra = reg_of_local[local_of[g->qa]], rb = reg_of_local[local_of[g->qb]];
+          std::memcpy(op.m, g->m, 32 * sizeof(double));
+          if (ra < rb) {
+            std::swap(ra, rb);
+            swap_2q_matrix(op.m);
+          }
+          op.a = ra;
+          op.b = rb;
+          op.cmask = g->cmask;
+          p.ops.push_back(op);
+          break;
+        }
+        case LGate::SIGN:
+          op.type = OP_SIGN;
+          op.cmask = g->sign_mask & ~rphys;
+          op.a = reg_mask(g->sign_mask);
+          p.ops.push_back(op);
+          break;
+        case LGate::DIAGX:
+          op.type = OP_DIAG;
+          op.ma = 1ull << g->qa;
+          op.mb = g->n_diag_targets == 2 ? (1ull << g->qb) : 0;
+          op.cmask = g->cmask;
+          std::memcpy(op.m, g->m, 8 * sizeof(double));
+          p.ops.push_back(op);
+          break;
+        case LGate::TERMS:
+          for (const auto& [mask, angle] : g->terms) {
+            PhaseTerm t{};
+            t.mask_out = mask & ~rphys;
+            t.rmask = reg_mask(mask);
+            t.angle = angle;
+            const int pc2 = popc(t.rmask);
+            t.cls = pc2 == 0 ? 0u : pc2 == 1 ? 1u + (uint32_t)__builtin_ctz(t.rmask) : pc2 == 2 ? pair_class(t.rmask) : 11u;
+            (t.cls == 11u ? pend_gen : pend_reg).push_back(t);
+          }
+          break;
+        default:
+          break;
+      }
+    }
+    flush_phase();
+    sd.op_end = (uint32_t)p.ops.size();
+    p.stages.push_back(sd);
+  }
+  Step s;
+  s.kind = Step::FUSED;
+  s.pass = (uint32_t)p.passes.size();
+  s.m = m;
+  s.n_stages = pd.n_stages;
+  p.passes.push_back(pd);
+  p.steps.push_back(s);
+}
+
+void plan_rule(Plan& p, std::vector<LGate> lg, uint32_t n, const PlanOptions& o) {
+  PlannerCtx pc;
+  pc.n = n;
+  const uint32_t mdef = p.precision == ALEPH_HIP_F64 ? 12u : 13u;
+  pc.m = std::min<uint32_t>(o.local_qubits ? o.local_qubits : mdef, n);
+  pc.m = std::min<uint32_t>(pc.m, kMaxLocalBits);
+  pc.L = std::min<uint32_t>(o.coalesce_bits ? o.coalesce_bits : 3u, pc.m - kRegBits);
+  pc.budget = o.max_ops ? (double)o.max_ops : 24.0;
+  const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
+  std::vector<char> done(lg.size(), 0);
+  size_t first = 0;
+  while (true) {
+    while (first < lg.size() && done[first]) first++;
+    if (first == lg.size()) break;
+    if (lg[first].standalone()) {
+      add_standalone(p, lg[first]);
+      done[first] = 1;
+      continue;
+    }
+    uint64_t S = (1ull << pc.L) - 1ull;
+    Blocker bl;
+    double cost = 0.0;
+    std::vector<const LGate*> take;
+    for (size_t i = first; i < lg.size(); i++) {
+      if (done[i]) continue;
+      const LGate& g = lg[i];
+      const uint64_t ndm = qmask(g.nd), dgm = qmask(g.dg);
+      bool ok = !g.standalone() && bl.admits(ndm, dgm) && popc(S | ndm) <= (int)pc.m;
+      if (ok && !take.empty() && cost + g.cost > pc.budget) {
+        // budget spent: close the pass here
+        break;
+      }
+      if (ok) {
+        take.push_back(&g);
+        done[i] = 1;
+        S |= ndm;
+        cost += g.cost;
+      } else {
+        bl.block(ndm, dgm);
+        if ((bl.any & all) == all) break;
+      }
+    }
+    // pad the chunk with the lowest unused qubits (longer contiguous runs)
+    for (uint32_t q = 0; q < n && popc(S) < (int)pc.m; q++) S |= 1ull << q;
+    emit_fused_pass(p, pc, S, take);
+  }
+}
+
+void plan_unfused(Plan& p, const std::vector<ProgInstr>& prog) {
+  for (const ProgInstr& pi : prog) {
+    if (pi.type == ProgInstr::BARRIER) continue;
+    Step s;
+    if (pi.type == ProgInstr::DIAG_PHASE) {
+      std::vector<double> ang = pi.angles;
+      std::vector<uint64_t> conds;
+      std::vector<uint32_t> off{0};
+      for (const auto& cs : pi.conds) {
+        for (uint64_t c : cs) conds.push_back(c);
+        off.push_back((uint32_t)conds.size());
+      }
+      s.kind = Step::POLY;
+      s.count = (uint32_t)ang.size();
+      s.dev_off = push_raw(p.payload, ang);
+      s.dev_off2 = push_raw(p.payload, conds);
+      s.dev_off3 = push_raw(p.payload, off);
+      p.steps.push_back(s);
+      continue;
+    }
+    const DenseGate& d = pi.gate;
+    const uint32_t k = (uint32_t)d.qubits.size();
+    s.cmask = qmask(d.controls);
+    s.k = k;
+    for (uint32_t j = 0; j < k && j < 8; j++) s.qubits[j] = d.qubits[j];
+    if (d.diagonal) {
+      std::vector<double> diag(2 * (size_t)d.dim);
+      for (uint32_t j = 0; j < d.dim; j++) {
+        diag[2 * j] = d.m[(size_t)j * d.dim + j].real();
+        diag[2 * j + 1] = d.m[(size_t)j * d.dim + j].imag();
+      }
+      s.kind = Step::DIAG;
+      s.dev_off = push_raw(p.payload, diag);
+    } else if (d.dim == 2 || d.dim == 4) {
+      s.kind = d.dim == 2 ? Step::G1Q : Step::G2Q;
+      for (uint32_t e = 0; e < d.dim * d.dim; e++) {
+        s.mat8[2 * e] = d.m[e].real();
+        s.mat8[2 * e + 1] = d.m[e].imag();
+      }
+    } else {
+      std::vector<double> mat(2 * (size_t)d.dim * d.dim);
+      for (size_t e = 0; e < (size_t)d.dim * d.dim; e++) {
+        mat[2 * e] = d.m[e].real();
+        mat[2 * e + 1] = d.m[e].imag();
+      }
+      s.kind = Step::KQ;
+      s.dev_off = p.precision == ALEPH_HIP_F64 ? push_matrix<double>(p.payload, mat)
+                                               : push_matrix<float>(p.payload, mat);
+    }
+    p.steps.push_back(s);
+  }
+}
+
+}  // namespace
+
+std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog, uint32_t n, uint32_t precision,
+                                 const PlanOptions& opts) {
+  const auto t0 = std::chrono::steady_clock::now();
+  auto p = std::make_unique<Plan>();
+  p->n = n;
+  p->precision = precision;
+  p->opts = opts;
+  uint32_t gates = 0;
+  for (const ProgInstr& pi : prog) gates += pi.type == ProgInstr::GATE ? 1 : 0;
+  if (opts.planner == ALEPH_HIP_PLANNER_UNFUSED || n < (uint32_t)kRegBits) {
+    plan_unfused(*p, prog);
+  } else {
+    std::vector<LGate> lg = lower_program(prog, n);
+    if (opts.fuse_1q) lg = fuse_1q_runs(lg);
+    plan_rule(*p, std::move(lg), n, opts);
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  aleph_hip_plan_stats& st = p->stats;
+  st.plan_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  st.input_gates = gates;
+  st.n_passes = (uint32_t)p->steps.size();
+  st.n_stages = (uint32_t)p->stages.size();
+  st.n_ops = (uint32_t)p->ops.size();
+  st.n_launches = (uint32_t)p->steps.size();
+  const double bpa = precision == ALEPH_HIP_F64 ? 16.0 : 8.0;
+  st.sweep_bytes = (double)st.n_passes * 2.0 * std::ldexp(bpa, (int)n);
+  return p;
+}
+
+Plan::~Plan() {
+  if (d_passes) (void)hipFree(d_passes);
+  if (d_stages) (void)hipFree(d_stages);
+  if (d_ops) (void)hipFree(d_ops);
+  if (d_terms) (void)hipFree(d_terms);
+  if (d_payload) (void)hipFree(d_payload);
+}
+
+namespace {
+template <typename V>
+void upload_vec(const std::vector<V>& v, V** dst, hipStream_t s) {
+  const size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(V);
+  ALEPH_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(dst), bytes));
+  if (!v.empty()) ALEPH_HIP_CHECK(hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice, s));
+}
+}  // namespace
+
+void Plan::upload(hipStream_t s) {
+  upload_vec(passes, &d_passes, s);
+  upload_vec(stages, &d_stages, s);
+  upload_vec(ops, &d_ops, s);
+  upload_vec(terms, &d_terms, s);
+  upload_vec(payload, &d_payload, s);
+  ALEPH_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template <typename T>
+static void execute_t(const Plan& p, hipStream_t s, void* state, hipEvent_t* ev) {
+  for (size_t i = 0; i < p.steps.size(); i++) {
+    const Step& st = p.steps[i];
+    if (ev) ALEPH_HIP_CHECK(hipEventRecord(ev[i], s));
+    switch (st.kind) {
+      case Step::FUSED:
+        launch_fused_pass<T>(s, state, p.n, st.m, st.n_stages, p.d_passes + st.pass, p.d_stages, p.d_ops, p.d_terms);
+        break;
+      case Step::KQ:
+        launch_kq<T>(s, state, p.n, st.k, st.qubits, st.cmask, p.d_payload + st.dev_off);
+        break;
+      case Step::DIAG:
+        launch_diag<T>(s, state, p.n, st.k, st.qubits, st.cmask,
+                       reinterpret_cast<const double*>(p.d_payload + st.dev_off));
+        break;
+      case Step::POLY:
+        launch_phase_poly<T>(s, state, p.n, st.count, reinterpret_cast<const double*>(p.d_payload + st.dev_off),
+                             reinterpret_cast<const uint64_t*>(p.d_payload + st.dev_off2),
+                             reinterpret_cast<const uint32_t*>(p.d_payload + st.dev_off3));
+        break;
+      case Step::G1Q:
+        launch_1q<T>(s, state, p.n, st.qubits[0], st.cmask, st.mat8);
+        break;
+      case Step::G2Q:
+        launch_2q<T>(s, state, p.n, st.qubits[0], st.qubits[1], st.cmask, st.mat8);
+        break;
+      case Step::SWAPQ:
+        launch_swap<T>(s, state, p.n, st.qubits[0], st.qubits[1], st.cmask);
+        break;
+    }
+  }
+  if (ev) ALEPH_HIP_CHECK(hipEventRecord(ev[p.steps.size()], s));
+}
+
+void Plan::execute(hipStream_t s, void* state) const {
+  if (precision == ALEPH_HIP_F64) execute_t<double>(*this, s, state, nullptr);
+  else execute_t<float>(*this, s, state, nullptr);
+}
+
+void Plan::execute_with_events(hipStream_t s, void* state, hipEvent_t* ev) const {
+  if (precision == ALEPH_HIP_F64) execute_t<double>(*this, s, state, ev);
+  else execute_t<float>(*this, s, state, ev);
+}
+
+void apply_dense_gate(hipStream_t s, void* state, uint32_t n, uint32_t precision, const DenseGate& d,
+                      void* scratch) {
+  const uint64_t cm = qmask(d.controls);
+  const uint32_t k = (uint32_t)d.qubits.size();
+  const bool f64 = precision == ALEPH_HIP_F64;
+  if (d.diagonal) {
+    std::vector<double> diag(2 * (size_t)d.dim);
+    for (uint32_t j = 0; j < d.dim; j++) {
+      diag[2 * j] = d.m[(size_t)j * d.dim + j].real();
+      diag[2 * j + 1] = d.m[(size_t)j * d.dim + j].imag();
+    }
+    ALEPH_HIP_CHECK(hipMemcpyAsync(scratch, diag.data(), diag.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    if (f64) launch_diag<double>(s, state, n, k, d.qubits.data(), cm, (const double*)scratch);
+    else launch_diag<float>(s, state, n, k, d.qubits.data(), cm, (const double*)scratch);
+    ALEPH_HIP_CHECK(hipStreamSynchronize(s));
+    return;
+  }
+  if (d.dim == 2 || d.dim == 4) {
+    double m[32];
+    for (uint32_t e = 0; e < d.dim * d.dim; e++) {
+      m[2 * e] = d.m[e].real();
+      m[2 * e + 1] = d.m[e].imag();
+    }
+    if (d.dim == 2) {
+      if (f64) launch_1q<double>(s, state, n, d.qubits[0], cm, m);
+      else launch_1q<float>(s, state, n, d.qubits[0], cm, m);
+    } else {
+      if (f64) launch_2q<double>(s, state, n, d.qubits[0], d.qubits[1], cm, m);
+      else launch_2q<float>(s, state, n, d.qubits[0], d.qubits[1], cm, m);
+    }
+    return;
+  }
+  std::vector<double> md(2 * (size_t)d.dim * d.dim);
+  for (size_t e = 0; e < (size_t)d.dim * d.dim; e++) {
+    md[2 * e] = d.m[e].real();
+    md[2 * e + 1] = d.m[e].imag();
+  }
+  if (f64) {
+    ALEPH_HIP_CHECK(hipMemcpyAsync(scratch, md.data(), md.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    launch_kq<double>(s, state, n, k, d.qubits.data(), cm, scratch);
+  } else {
+    std::vector<float> mf(md.begin(), md.end());
+    ALEPH_HIP_CHECK(hipMemcpyAsync(scratch, mf.data(), mf.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    launch_kq<float>(s, state, n, k, d.qubits.data(), cm, scratch);
+  }
+  ALEPH_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace aleph

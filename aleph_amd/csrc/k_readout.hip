@@ -1,0 +1,377 @@
+// Readout reductions: norm / branch probability, collapse, Pauli
+// expectation, marginals and shot sampling.  All accumulation is FP64
+// (also for complex64 states, as crates/aleph-cuda/src/sv/readout_f32.cu:1-8
+// requires); per-thread sums are Neumaier-compensated and combined with
+// error-free two-sums across lanes and workgroups.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace aleph {
+namespace {
+
+constexpr uint32_t kThreads = 256;
+
+struct KSum {
+  double s, c;
+};
+
+__device__ __forceinline__ void kadd(KSum& k, double x) {
+  const double t = k.s + x;
+  if (fabs(k.s) >= fabs(x)) k.c += (k.s - t) + x;
+  else k.c += (x - t) + k.s;
+  k.s = t;
+}
+
+__device__ __forceinline__ KSum kmerge(KSum a, KSum b) {
+  KSum r = a;
+  kadd(r, b.s);
+  r.c += b.c;
+  return r;
+}
+
+__device__ __forceinline__ KSum wave_reduce(KSum v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    KSum o;
+    o.s = __shfl_down(v.s, off, 64);
+    o.c = __shfl_down(v.c, off, 64);
+    v = kmerge(v, o);
+  }
+  return v;
+}
+
+// Block-wide reduction of up to 3 compensated sums; result valid in thread 0.
+template <int N>
+__device__ void block_reduce(KSum (&v)[N]) {
+  __shared__ KSum sh[N][kThreads / 64];
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < N; i++) v[i] = wave_reduce(v[i]);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < N; i++) sh[i][wid] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      KSum acc = sh[i][0];
+      for (uint32_t w = 1; w < blockDim.x / 64; w++) acc = kmerge(acc, sh[i][w]);
+      v[i] = acc;
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_reduce_branch(const C2<T>* __restrict__ s, uint64_t len,
+                                                            uint64_t qmask, double* __restrict__ partials) {
+  KSum v[3] = {{0, 0}, {0, 0}, {0, 0}};
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
+    const C2<T> a = s[i];
+    const double p = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+    if (!isfinite(p)) {
+      v[2].s += 1.0;
+      continue;
+    }
+    if (i & qmask) kadd(v[1], p);
+    else kadd(v[0], p);
+  }
+  block_reduce<3>(v);
+  if (threadIdx.x == 0) {
+    partials[3 * blockIdx.x + 0] = v[0].s + v[0].c;
+    partials[3 * blockIdx.x + 1] = v[1].s + v[1].c;
+    partials[3 * blockIdx.x + 2] = v[2].s;
+  }
+}
+
+// Sums `count` interleaved groups of `width` partials into out[0..width).
+__global__ void __launch_bounds__(kThreads) k_reduce_final(const double* __restrict__ partials, uint32_t count,
+                                                           uint32_t width, double* __restrict__ out) {
+  KSum v[3] = {{0, 0}, {0, 0}, {0, 0}};
+  for (uint32_t b = threadIdx.x; b < count; b += blockDim.x)
+    for (uint32_t w = 0; w < width && w < 3; w++) kadd(v[w], partials[width * b + w]);
+  block_reduce<3>(v);
+  if (threadIdx.x == 0)
+    for (uint32_t w = 0; w < width && w < 3; w++) out[w] = v[w].s + v[w].c;
+}
+
+template <typename T>
+__global__ void k_collapse(C2<T>* __restrict__ s, uint64_t len, uint64_t qmask, int32_t outcome, double scale) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  const bool keep = ((i & qmask) != 0) == (outcome != 0);
+  C2<T> a = s[i];
+  if (keep) {
+    a.x = (T)((double)a.x * scale);
+    a.y = (T)((double)a.y * scale);
+  } else {
+    a.x = 0;
+    a.y = 0;
+  }
+  s[i] = a;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_expect_pauli(const C2<T>* __restrict__ s, uint64_t len, uint64_t x,
+                                                           uint64_t zy, double* __restrict__ partials) {
+  KSum v[2] = {{0, 0}, {0, 0}};
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
+    const C2<T> a = s[i], b = s[i ^ x];
+    const double sg = (__popcll(i & zy) & 1) ? -1.0 : 1.0;
+    // conj(b) * a
+    kadd(v[0], sg * ((double)b.x * (double)a.x + (double)b.y * (double)a.y));
+    kadd(v[1], sg * ((double)b.x * (double)a.y - (double)b.y * (double)a.x));
+  }
+  block_reduce<2>(v);
+  if (threadIdx.x == 0) {
+    partials[3 * blockIdx.x + 0] = v[0].s + v[0].c;
+    partials[3 * blockIdx.x + 1] = v[1].s + v[1].c;
+    partials[3 * blockIdx.x + 2] = 0.0;
+  }
+}
+
+struct Qubits64 { uint32_t q[64]; };
+
+// LDS-privatised histogram (k <= 12); wider marginals go straight to global atomics.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_marginal(const C2<T>* __restrict__ s, uint64_t len, uint32_t k,
+                                                       Qubits64 qs, double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* h = reinterpret_cast<double*>(smem);
+  const uint32_t bins = 1u << k;
+  const bool priv = k <= 12;
+  if (priv) {
+    for (uint32_t b = threadIdx.x; b < bins; b += blockDim.x) h[b] = 0.0;
+    __syncthreads();
+  }
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
+    const C2<T> a = s[i];
+    const double p = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+    uint32_t bin = 0;
+    for (uint32_t j = 0; j < k; j++) bin |= (uint32_t)((i >> qs.q[j]) & 1ull) << j;
+    if (priv) atomicAdd(&h[bin], p);
+    else atomicAdd(&out[bin], p);
+  }
+  if (priv) {
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < bins; b += blockDim.x)
+      if (h[b] != 0.0) atomicAdd(&out[b], h[b]);
+  }
+}
+
+__global__ void k_zero(double* __restrict__ p, uint64_t count) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0.0;
+}
+
+// ---- sampler ----
+constexpr uint32_t kSampleBlock = 1u << kSampleBlockBits;
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_block_sums(const C2<T>* __restrict__ s, uint64_t len,
+                                                         double* __restrict__ sums, double* __restrict__ prefix) {
+  const uint64_t base = (uint64_t)blockIdx.x * kSampleBlock;
+  const uint32_t per = kSampleBlock / kThreads;
+  KSum v[1] = {{0, 0}};
+  const uint64_t lim = len < base + kSampleBlock ? len : base + kSampleBlock;
+  for (uint32_t j = 0; j < per; j++) {
+    const uint64_t i = base + (uint64_t)j * kThreads + threadIdx.x;
+    if (i < lim) {
+      const C2<T> a = s[i];
+      kadd(v[0], (double)a.x * (double)a.x + (double)a.y * (double)a.y);
+    }
+  }
+  block_reduce<1>(v);
+  if (threadIdx.x == 0) {
+    const double t = v[0].s + v[0].c;
+    sums[blockIdx.x] = t;
+    prefix[blockIdx.x] = t;
+  }
+}
+
+// Exclusive prefix of `count` sums in place, one workgroup of 1024 threads;
+// the grand total lands in total[0].
+__global__ void __launch_bounds__(1024) k_scan(double* __restrict__ v, uint64_t count, double* __restrict__ total) {
+  __shared__ double part[1024];
+  const uint32_t t = threadIdx.x, nt = blockDim.x;
+  const uint64_t per = (count + nt - 1) / nt;
+  const uint64_t b = t * per, e = b + per < count ? b + per : count;
+  double acc = 0.0;
+  for (uint64_t i = b; i < e; i++) acc += v[i];
+  part[t] = acc;
+  __syncthreads();
+  if (t == 0) {
+    double run = 0.0;
+    for (uint32_t j = 0; j < nt; j++) {
+      const double x = part[j];
+      part[j] = run;
+      run += x;
+    }
+    total[0] = run;
+  }
+  __syncthreads();
+  double run = part[t];
+  for (uint64_t i = b; i < e; i++) {
+    const double x = v[i];
+    v[i] = run;
+    run += x;
+  }
+}
+
+// One wavefront per shot: binary search over block prefixes, then a
+// lane-parallel scan inside the chosen block.  Zero-probability amplitudes
+// are never returned (strict `target < cumulative`, as
+// crates/aleph-sv/src/sampling.rs:61-83 does).
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_sample(const C2<T>* __restrict__ s, uint64_t len,
+                                                     const double* __restrict__ prefix, const double* __restrict__ sums,
+                                                     uint64_t nblocks, const double* __restrict__ total,
+                                                     const double* __restrict__ uni, uint32_t shots,
+                                                     uint64_t* __restrict__ out) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t shot = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (shot >= shots) return;
+  const double target = uni[shot] * total[0];
+  // last block whose prefix <= target among blocks with positive mass
+  uint64_t lo = 0, hi = nblocks;  // invariant: answer in [lo, hi)
+  while (hi - lo > 1) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (prefix[mid] <= target) lo = mid;
+    else hi = mid;
+  }
+  uint64_t blk = lo;
+  while (blk > 0 && sums[blk] <= 0.0) blk--;          // never land in an empty block
+  while (blk + 1 < nblocks && sums[blk] <= 0.0) blk++;
+  const double local = target - prefix[blk];
+  const uint64_t base = blk * kSampleBlock;
+  const uint32_t per = kSampleBlock / 64;             // amplitudes per lane
+  double mine = 0.0;
+  for (uint32_t j = 0; j < per; j++) {
+    const uint64_t i = base + (uint64_t)lane * per + j;
+    if (i < len) {
+      const C2<T> a = s[i];
+      mine += (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+    }
+  }
+  // inclusive scan over lanes
+  double incl = mine;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double o = __shfl_up(incl, off, 64);
+    if ((int)lane >= off) incl += o;
+  }
+  const double excl = incl - mine;
+  const bool hit = mine > 0.0 && local < incl && local >= excl;
+  const uint64_t hits = __ballot(hit);
+  const uint64_t pos_lanes = __ballot(mine > 0.0);
+  int who;
+  if (hits) who = __ffsll((long long)hits) - 1;
+  else who = pos_lanes ? 63 - __clzll((long long)pos_lanes) : 0;   // rounding overflow: last positive lane
+  if ((int)lane == who) {
+    const double rem = hits ? local - excl : 1e308;
+    double cum = 0.0;
+    uint64_t pick = base + (uint64_t)lane * per;
+    uint64_t last_pos = pick;
+    bool found = false;
+    for (uint32_t j = 0; j < per; j++) {
+      const uint64_t i = base + (uint64_t)lane * per + j;
+      if (i >= len) break;
+      const C2<T> a = s[i];
+      const double p = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+      if (p > 0.0) {
+        last_pos = i;
+        cum += p;
+        if (!found && rem < cum) {
+          pick = i;
+          found = true;
+        }
+      }
+    }
+    out[shot] = found ? pick : last_pos;
+  }
+}
+
+inline uint32_t grid_for(uint64_t items) {
+  const uint64_t g = (items + kThreads - 1) / kThreads;
+  return (uint32_t)(g == 0 ? 1 : g);
+}
+
+}  // namespace
+
+template <typename T>
+void launch_reduce_branch(hipStream_t st, const void* state, uint32_t n, int32_t qubit, double* partials,
+                          double* out) {
+  const uint64_t len = 1ull << n;
+  const uint64_t qmask = qubit >= 0 ? (1ull << qubit) : 0ull;
+  uint32_t blocks = grid_for(len);
+  if (blocks > kReduceBlocks) blocks = kReduceBlocks;
+  hipLaunchKernelGGL(k_reduce_branch<T>, dim3(blocks), dim3(kThreads), 0, st, (const C2<T>*)state, len, qmask,
+                     partials);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, st, partials, blocks, 3u, out);
+  ALEPH_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_collapse(hipStream_t st, void* state, uint32_t n, uint32_t qubit, int32_t outcome, double scale) {
+  const uint64_t len = 1ull << n;
+  hipLaunchKernelGGL(k_collapse<T>, dim3(grid_for(len)), dim3(kThreads), 0, st, (C2<T>*)state, len, 1ull << qubit,
+                     outcome, scale);
+  ALEPH_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_expect_pauli(hipStream_t st, const void* state, uint32_t n, uint64_t xmask, uint64_t zymask,
+                         double* partials, double* out) {
+  const uint64_t len = 1ull << n;
+  uint32_t blocks = grid_for(len);
+  if (blocks > kReduceBlocks) blocks = kReduceBlocks;
+  hipLaunchKernelGGL(k_expect_pauli<T>, dim3(blocks), dim3(kThreads), 0, st, (const C2<T>*)state, len, xmask,
+                     zymask, partials);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, st, partials, blocks, 3u, out);
+  ALEPH_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_marginal(hipStream_t st, const void* state, uint32_t n, uint32_t k, const uint32_t* qubits, double* out) {
+  const uint64_t len = 1ull << n;
+  const uint64_t bins = 1ull << k;
+  hipLaunchKernelGGL(k_zero, dim3(grid_for(bins)), dim3(kThreads), 0, st, out, bins);
+  Qubits64 qs{};
+  for (uint32_t j = 0; j < k && j < 64; j++) qs.q[j] = qubits[j];
+  uint32_t blocks = grid_for(len);
+  if (blocks > kReduceBlocks) blocks = kReduceBlocks;
+  const size_t lds = k <= 12 ? sizeof(double) * bins : 0;
+  hipLaunchKernelGGL(k_marginal<T>, dim3(blocks), dim3(kThreads), lds, st, (const C2<T>*)state, len, k, qs, out);
+  ALEPH_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_sample(hipStream_t st, const void* state, uint32_t n, double* block_sums, double* scan_tmp,
+                   const double* uniforms, uint32_t shots, uint64_t* out) {
+  const uint64_t len = 1ull << n;
+  const uint64_t nb = n > kSampleBlockBits ? (1ull << (n - kSampleBlockBits)) : 1ull;
+  double* sums = block_sums;       // nb
+  double* prefix = block_sums + nb;  // nb
+  double* total = scan_tmp;
+  hipLaunchKernelGGL(k_block_sums<T>, dim3((uint32_t)nb), dim3(kThreads), 0, st, (const C2<T>*)state, len, sums,
+                     prefix);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, prefix, nb, total);
+  if (shots) {
+    const uint64_t threads = (uint64_t)shots * 64;
+    hipLaunchKernelGGL(k_sample<T>, dim3(grid_for(threads)), dim3(kThreads), 0, st, (const C2<T>*)state, len, prefix,
+                       sums, nb, total, uniforms, shots, out);
+  }
+  ALEPH_HIP_CHECK(hipGetLastError());
+}
+
+#define ALEPH_INST(T)                                                                                           \
+  template void launch_reduce_branch<T>(hipStream_t, const void*, uint32_t, int32_t, double*, double*);         \
+  template void launch_collapse<T>(hipStream_t, void*, uint32_t, uint32_t, int32_t, double);                    \
+  template void launch_expect_pauli<T>(hipStream_t, const void*, uint32_t, uint64_t, uint64_t, double*, double*); \
+  template void launch_marginal<T>(hipStream_t, const void*, uint32_t, uint32_t, const uint32_t*, double*);      \
+  template void launch_sample<T>(hipStream_t, const void*, uint32_t, double*, double*, const double*, uint32_t,   \
+                                 uint64_t*);
+ALEPH_INST(double)
+ALEPH_INST(float)
+#undef ALEPH_INST
+
+}  // namespace aleph

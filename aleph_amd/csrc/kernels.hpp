@@ -1,0 +1,69 @@
+// Host-callable launchers of the gfx950 kernels.  T is double or float
+// (complex128 / complex64 amplitudes); every launch is ordered on `stream`.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "plan.hpp"
+
+namespace aleph {
+
+// ---- state preparation and per-gate kernels (k_gates.hip) ----
+template <typename T> void launch_init_basis(hipStream_t s, void* state, uint32_t n);
+template <typename T>
+void launch_1q(hipStream_t s, void* state, uint32_t n, uint32_t target, uint64_t cmask, const double* m8);
+template <typename T>
+void launch_2q(hipStream_t s, void* state, uint32_t n, uint32_t q_msb, uint32_t q_lsb, uint64_t cmask,
+               const double* m32);
+// Dense 2^k x 2^k (k <= 6) on qubits (MSB first); `dmat` is a device copy of
+// the interleaved matrix in the state's precision, row-major.
+template <typename T>
+void launch_kq(hipStream_t s, void* state, uint32_t n, uint32_t k, const uint32_t* qubits, uint64_t cmask,
+               const void* dmat);
+// Diagonal on k qubits (MSB first); `ddiag` device copy of 2^k complex in double.
+template <typename T>
+void launch_diag(hipStream_t s, void* state, uint32_t n, uint32_t k, const uint32_t* qubits, uint64_t cmask,
+                 const double* ddiag);
+template <typename T>
+void launch_swap(hipStream_t s, void* state, uint32_t n, uint32_t a, uint32_t b, uint64_t cmask);
+// Parity phase polynomial (DiagonalPhase semantics), CSR on the device.
+template <typename T>
+void launch_phase_poly(hipStream_t s, void* state, uint32_t n, uint32_t n_terms, const double* angles,
+                       const uint64_t* conds, const uint32_t* offsets);
+template <typename T>
+void launch_permute(hipStream_t s, const void* src, void* dst, uint32_t n, const uint32_t* perm);
+void launch_widen_f32(hipStream_t s, const void* src_f32, void* dst_f64, uint64_t len);
+void launch_narrow_f64(hipStream_t s, const void* src_f64, void* dst_f32, uint64_t len);
+
+// ---- fused pass (k_fused.hip) ----
+template <typename T>
+void launch_fused_pass(hipStream_t s, void* state, uint32_t n, uint32_t m, uint32_t n_stages, const PassDev* pd,
+                       const StageDev* stages, const DevOp* ops, const PhaseTerm* terms);
+
+// ---- readout (k_readout.hip) ----
+// Result layout in `out` (device, 3 doubles): [sum of |a|^2 over bit q clear,
+// sum over bit q set, non-finite count]; qubit < 0 puts everything in [0].  `partials` needs 3 * kReduceBlocks doubles.
+constexpr uint32_t kReduceBlocks = 1024;
+template <typename T>
+void launch_reduce_branch(hipStream_t s, const void* state, uint32_t n, int32_t qubit, double* partials,
+                          double* out);
+template <typename T>
+void launch_collapse(hipStream_t s, void* state, uint32_t n, uint32_t qubit, int32_t outcome, double scale);
+// <psi| P |psi> pieces: out[0] = Re, out[1] = Im of sum conj(psi[i^x]) * (-1)^{|i & zy|} psi[i].
+template <typename T>
+void launch_expect_pauli(hipStream_t s, const void* state, uint32_t n, uint64_t xmask, uint64_t zymask,
+                         double* partials, double* out);
+// Marginal histogram over k qubits into `out` (2^k doubles, device, zeroed by the launcher).
+template <typename T>
+void launch_marginal(hipStream_t s, const void* state, uint32_t n, uint32_t k, const uint32_t* qubits,
+                     double* out);
+// Sampler: block sums (2^(n - kSampleBlockBits) doubles, scanned in place into
+// an exclusive prefix), then one wavefront per shot descends.
+constexpr uint32_t kSampleBlockBits = 12;
+template <typename T>
+void launch_sample(hipStream_t s, const void* state, uint32_t n, double* block_sums, double* scan_tmp,
+                   const double* uniforms, uint32_t shots, uint64_t* out);
+
+}  // namespace aleph

@@ -1,0 +1,74 @@
+// Device-visible descriptors of an execution plan.
+//
+// A plan is a list of full-state sweeps ("passes").  A fused pass owns a
+// set S of m physical qubits (the chunk qubits); each workgroup streams one
+// chunk of 2^m amplitudes (all other index bits fixed) HBM -> registers,
+// applies every op of the pass, and writes the chunk back once.  Inside a
+// pass the chunk moves through "stages": in a stage every thread holds 2^G
+// amplitudes in registers spanning G of the chunk qubits (the register
+// qubits); dense gates act on register qubits only, diagonal ops and
+// controls act on any qubit through the amplitude's global index.  Between
+// stages the chunk is re-distributed through LDS.
+#pragma once
+
+#include <cstdint>
+
+namespace aleph {
+
+constexpr int kRegBits = 4;                 // G: amplitudes per thread = 16
+constexpr int kRegAmps = 1 << kRegBits;
+constexpr int kMaxLocalBits = 14;           // chunk <= 2^14 amplitudes (1024 threads)
+constexpr int kMaxThreadBits = kMaxLocalBits - kRegBits;
+
+enum OpType : uint32_t {
+  OP_1Q = 1,     // 2x2 on register bit a; optional control mask (global index)
+  OP_2Q = 2,     // 4x4 on register bits (a = matrix MSB, b = LSB), a > b
+  OP_DIAG = 3,   // diagonal on up to two qubits given as global masks ma (MSB), mb
+  OP_SIGN = 4,   // negate where (idx & mask) == mask; thread part in `cmask`, register part in `a`
+  OP_PHASE = 5,  // multiply by exp(i*phi): terms[term_off, +n_terms) of classes 0..10, then `b` class-11 terms
+};
+
+struct alignas(16) DevOp {
+  uint32_t type;
+  uint32_t a, b;
+  uint32_t n_terms;
+  uint64_t cmask;     // OP_1Q/2Q/DIAG: controls (global); OP_SIGN: thread-side mask
+  uint64_t ma, mb;    // OP_DIAG target masks
+  uint32_t term_off;
+  uint32_t pad;
+  double m[32];       // OP_1Q: 2x2, OP_2Q: 4x4, OP_DIAG: 4 entries (interleaved complex)
+};
+
+// Phase term: contributes `angle` when (thread_index & mask_out) == mask_out
+// and (reg & rmask) == rmask.  `cls` selects the accumulator:
+// 0 constant, 1..4 linear in register bit cls-1, 5..10 register pair,
+// 11 general (three or four register bits).
+struct alignas(16) PhaseTerm {
+  uint64_t mask_out;
+  double angle;
+  uint32_t cls;
+  uint32_t rmask;
+  uint32_t pad[2];
+};
+
+struct alignas(16) StageDev {
+  uint32_t op_begin, op_end;
+  uint32_t n_thread_bits;
+  uint32_t pad;
+  uint64_t tg[kMaxThreadBits];   // physical mask of thread-id bit j
+  uint32_t tl[kMaxThreadBits];   // chunk-local mask of thread-id bit j
+  uint32_t pad2[2];
+  uint64_t rg[kRegBits];         // physical mask of register bit g
+  uint32_t rl[kRegBits];         // chunk-local mask of register bit g
+};
+
+struct alignas(16) PassDev {
+  uint32_t m;                    // chunk qubits
+  uint32_t n_stages;
+  uint32_t stage_begin;
+  uint32_t pad;
+  uint32_t local_pos[kMaxLocalBits];  // ascending physical positions of chunk qubits
+  uint32_t pad2[2];
+};
+
+}  // namespace aleph

@@ -1,0 +1,234 @@
+/*
+ * aleph_hip.h — C ABI of the MI355X (gfx950) dense state-vector backend.
+ *
+ * This is the boundary a Rust `aleph-hip` crate binds with `extern "C"`
+ * to implement `aleph_backend::Backend` (crates/aleph-backend/src/lib.rs:86-163)
+ * and a `CudaPlanBackend`-style plan hook (crates/aleph-cuda/src/execution.rs:197-208),
+ * so that `--backend hip` / `backend="hip"` sits beside `sv` and `cuda`.
+ * Every entry point takes plain pointers and sizes and returns an
+ * `aleph_hip_status` whose values map one-to-one onto `BackendError`
+ * variants (crates/aleph-backend/src/lib.rs:22-81).  Structured error
+ * detail (qubit, limits, deviation, gate name) is read back with
+ * aleph_hip_last_error().
+ *
+ * Conventions carried across the boundary (same as the reference):
+ *  - amplitude index bit q is qubit q (qubit 0 = LSB), ADR 0004 as
+ *    implemented by crates/aleph-sv/src/kernels/aos.rs:1-8;
+ *  - multi-qubit matrices are row-major with qubits[0] as the MSB of the
+ *    row/column index (crates/aleph-core/src/gate/kinds.rs:74-96);
+ *  - complex numbers are interleaved (re, im) doubles on the host side;
+ *  - one caller at a time per context (Backend methods take &mut self);
+ *    launches are stream-ordered, readout calls synchronise;
+ *  - randomness lives on the host: the context owns a seeded generator
+ *    and draws one f64 per sampled shot and one f64 per genuine
+ *    (non-degenerate) measurement, the consumption contract of
+ *    crates/aleph-cuda/src/sv/sampling.rs:209 and
+ *    crates/aleph-sv/src/measure.rs:79-90.
+ */
+#ifndef ALEPH_HIP_H
+#define ALEPH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (BackendError, crates/aleph-backend/src/lib.rs:22-81) ---- */
+typedef int32_t aleph_hip_status;
+enum {
+  ALEPH_HIP_OK = 0,
+  ALEPH_HIP_ERR_DEVICE_FAILURE = 1,         /* DeviceFailure(String)            */
+  ALEPH_HIP_ERR_QUBIT_OUT_OF_RANGE = 2,     /* QubitOutOfRange{qubit,num_qubits} */
+  ALEPH_HIP_ERR_DUPLICATE_QUBIT = 3,        /* DuplicateQubit{qubit}            */
+  ALEPH_HIP_ERR_ARITY_MISMATCH = 4,         /* ArityMismatch{kind,expected,got} */
+  ALEPH_HIP_ERR_UNSUPPORTED_GATE = 5,       /* UnsupportedGate{kind}            */
+  ALEPH_HIP_ERR_UNSUPPORTED_INSTRUCTION = 6,/* UnsupportedInstruction{kind}     */
+  ALEPH_HIP_ERR_SYMBOLIC_PARAM = 7,         /* SymbolicParam                    */
+  ALEPH_HIP_ERR_NON_FINITE_PARAM = 8,       /* NonFiniteParam{kind}             */
+  ALEPH_HIP_ERR_NON_UNITARY_MATRIX = 9,     /* NonUnitaryMatrix{deviation}      */
+  ALEPH_HIP_ERR_EMPTY_CIRCUIT = 10,         /* EmptyCircuit                     */
+  ALEPH_HIP_ERR_DEGENERATE_MEASUREMENT = 11,/* DegenerateMeasurement{qubit,p}   */
+  ALEPH_HIP_ERR_TOO_MANY_QUBITS = 12,       /* TooManyQubits{requested,limit}   */
+  ALEPH_HIP_ERR_INVALID_PAULI_STRING = 13,  /* InvalidPauliString{reason}       */
+  ALEPH_HIP_ERR_INVALID_STATE = 14,         /* InvalidState{reason}             */
+  ALEPH_HIP_ERR_OPTIMIZATION = 15,          /* Optimization(PassError)          */
+  ALEPH_HIP_ERR_NO_DEVICE = 16,             /* aleph-cuda Error::NoDevice       */
+  ALEPH_HIP_ERR_INVALID_ARGUMENT = 17,      /* null pointer / bad size: DeviceFailure */
+  ALEPH_HIP_ERR_OUT_OF_MEMORY = 18          /* memory preflight rejection: DeviceFailure */
+};
+
+/* ---- gate kinds: order of the reference `Gate` enum (kinds.rs:13-115) ---- */
+enum {
+  ALEPH_HIP_GATE_H = 0, ALEPH_HIP_GATE_X, ALEPH_HIP_GATE_Y, ALEPH_HIP_GATE_Z,
+  ALEPH_HIP_GATE_S, ALEPH_HIP_GATE_SDG, ALEPH_HIP_GATE_T, ALEPH_HIP_GATE_TDG,
+  ALEPH_HIP_GATE_RX, ALEPH_HIP_GATE_RY, ALEPH_HIP_GATE_RZ, ALEPH_HIP_GATE_PHASE,
+  ALEPH_HIP_GATE_U3,
+  ALEPH_HIP_GATE_CNOT, ALEPH_HIP_GATE_CZ, ALEPH_HIP_GATE_SWAP, ALEPH_HIP_GATE_ISWAP,
+  ALEPH_HIP_GATE_ISWAPDG, ALEPH_HIP_GATE_CRX, ALEPH_HIP_GATE_CRY, ALEPH_HIP_GATE_CRZ,
+  ALEPH_HIP_GATE_TOFFOLI, ALEPH_HIP_GATE_CCZ,
+  ALEPH_HIP_GATE_UNITARY1Q, ALEPH_HIP_GATE_UNITARY2Q, ALEPH_HIP_GATE_UNITARYKQ,
+  ALEPH_HIP_GATE_UNITARY1QDIAG, ALEPH_HIP_GATE_UNITARY2QDIAG,
+  ALEPH_HIP_GATE_COUNT
+};
+
+/* Program-only instruction records (Instruction enum, crates/aleph-ir/src/instruction.rs:13-36). */
+enum {
+  ALEPH_HIP_INSTR_DIAGONAL_PHASE = 64,
+  ALEPH_HIP_INSTR_BARRIER = 65
+};
+
+enum { ALEPH_HIP_F64 = 64, ALEPH_HIP_F32 = 32 };
+
+typedef struct aleph_hip_ctx aleph_hip_ctx;
+typedef struct aleph_hip_state aleph_hip_state;
+typedef struct aleph_hip_plan aleph_hip_plan;
+
+typedef struct aleph_hip_error {
+  int32_t code;
+  uint32_t qubit;       /* QubitOutOfRange / DuplicateQubit / DegenerateMeasurement */
+  uint32_t num_qubits;  /* QubitOutOfRange */
+  uint32_t expected;    /* ArityMismatch.expected, TooManyQubits.limit */
+  uint32_t got;         /* ArityMismatch.got, TooManyQubits.requested */
+  double value;         /* NonUnitaryMatrix.deviation, DegenerateMeasurement.probability */
+  char kind[48];        /* gate / instruction name, or the InvalidState reason */
+  char message[256];    /* rendered message, device errors included */
+} aleph_hip_error;
+
+/* One gate placed on qubits: GateInstance (crates/aleph-core/src/gate/instance.rs:40-59).
+ * `params` holds the gate's angles (U3: theta, phi, lambda); `matrix` the
+ * interleaved payload of Unitary1q/2q/Kq (dim*dim complex, row-major) or of
+ * Unitary1qDiag/2qDiag (dim complex).  `k` is the width of UnitaryKq. */
+typedef struct aleph_hip_gate {
+  uint32_t kind;
+  uint32_t n_qubits;
+  uint32_t n_controls;
+  uint32_t k;
+  const uint32_t* qubits;
+  const uint32_t* controls;
+  double params[3];
+  const double* matrix;
+} aleph_hip_gate;
+
+/* ---- context: replaces CudaSvBackend::with_seed (crates/aleph-cuda/src/sv/backend.rs:88-129) ---- */
+aleph_hip_status aleph_hip_device_count(int32_t* count);
+aleph_hip_status aleph_hip_ctx_create(int32_t ordinal, uint64_t seed, aleph_hip_ctx** out);
+void aleph_hip_ctx_destroy(aleph_hip_ctx* ctx);
+/* with_qubit_cap (crates/aleph-sv/src/backend.rs:46-50); default 32 for F64, 33 for F32 */
+aleph_hip_status aleph_hip_ctx_set_qubit_cap(aleph_hip_ctx* ctx, uint32_t cap);
+aleph_hip_status aleph_hip_ctx_reseed(aleph_hip_ctx* ctx, uint64_t seed);
+aleph_hip_status aleph_hip_synchronize(aleph_hip_ctx* ctx);
+aleph_hip_status aleph_hip_last_error(const aleph_hip_ctx* ctx, aleph_hip_error* out);
+/* The hipStream_t every launch of this context is ordered on (for event timing). */
+aleph_hip_status aleph_hip_ctx_stream(aleph_hip_ctx* ctx, void** stream);
+
+/* ---- state: Backend::allocate (lib.rs:95) / CudaSvState (crates/aleph-cuda/src/sv/state.rs) ---- */
+aleph_hip_status aleph_hip_state_alloc(aleph_hip_ctx* ctx, uint32_t num_qubits, uint32_t precision,
+                                       aleph_hip_state** out);
+void aleph_hip_state_free(aleph_hip_state* st);
+/* Resets to |0...0>. */
+aleph_hip_status aleph_hip_state_reset(aleph_hip_ctx* ctx, aleph_hip_state* st);
+/* Host interleaved complex128 in / out (FP32 states narrow / widen). */
+aleph_hip_status aleph_hip_state_load(aleph_hip_ctx* ctx, aleph_hip_state* st, const double* amps);
+aleph_hip_status aleph_hip_state_read(aleph_hip_ctx* ctx, aleph_hip_state* st, double* out);
+/* Device-resident copy in / out in the state's own precision. */
+aleph_hip_status aleph_hip_state_load_device(aleph_hip_ctx* ctx, aleph_hip_state* st, const void* dptr);
+aleph_hip_status aleph_hip_state_read_device(aleph_hip_ctx* ctx, aleph_hip_state* st, void* dptr);
+aleph_hip_status aleph_hip_state_info(const aleph_hip_state* st, uint32_t* num_qubits, uint32_t* precision,
+                                      void** device_ptr);
+
+/* ---- gates: Backend::apply_gate (crates/aleph-sv/src/backend.rs:74-151) ---- */
+aleph_hip_status aleph_hip_apply_gate(aleph_hip_ctx* ctx, aleph_hip_state* st, const aleph_hip_gate* g);
+/* Backend::apply_diagonal_phase (lib.rs:128-138; crates/aleph-sv/src/kernels/diagonal_phase.rs:26-45).
+ * CSR terms: term t fires when popcount(conds[c] & x) is odd for every
+ * c in [offsets[t], offsets[t+1]), adding angles[t] to the phase. */
+aleph_hip_status aleph_hip_apply_diagonal_phase(aleph_hip_ctx* ctx, aleph_hip_state* st, uint32_t n_terms,
+                                                const double* angles, const uint64_t* conds,
+                                                const uint32_t* offsets);
+/* Backend::unpermute_state (lib.rs:156-163; crates/aleph-core/src/perm.rs:25-40), perm[logical] = physical. */
+aleph_hip_status aleph_hip_unpermute(aleph_hip_ctx* ctx, aleph_hip_state* st, const uint32_t* perm);
+
+/* Low-level kernels (CUDA peers in crates/aleph-cuda/src/sv/kernels.cu, diag.cu), no validation
+ * beyond range checks.  Masks are u64. */
+aleph_hip_status aleph_hip_apply_1q(aleph_hip_ctx* ctx, aleph_hip_state* st, uint32_t target,
+                                    uint64_t ctrl_mask, const double* m /* 8 doubles */);
+aleph_hip_status aleph_hip_apply_kq(aleph_hip_ctx* ctx, aleph_hip_state* st, uint32_t k, const uint32_t* qubits,
+                                    uint64_t ctrl_mask, const double* m /* 2*4^k doubles */);
+aleph_hip_status aleph_hip_apply_diag(aleph_hip_ctx* ctx, aleph_hip_state* st, uint32_t k, const uint32_t* qubits,
+                                      uint64_t ctrl_mask, const double* d /* 2*2^k doubles */);
+aleph_hip_status aleph_hip_apply_cnot(aleph_hip_ctx* ctx, aleph_hip_state* st, uint32_t control, uint32_t target);
+aleph_hip_status aleph_hip_apply_swap(aleph_hip_ctx* ctx, aleph_hip_state* st, uint32_t a, uint32_t b);
+
+/* ---- programs: CudaExecutionPlan::compile / execute (crates/aleph-cuda/src/execution.rs:78-194) ----
+ * A program is a flat instruction stream.  `words` per record:
+ *   gate:           kind, n_qubits, n_controls, k, qubits..., controls...
+ *   diagonal phase: 64, n_terms, then per term: n_conds, (lo32, hi32) per cond
+ *   barrier:        65, n, qubits...
+ * `reals` per record, in the same order: the gate's params (1 for the
+ * one-angle kinds, 3 for U3, none otherwise) followed by its matrix payload
+ * (Unitary* kinds); a diagonal phase contributes one angle per term.
+ */
+enum { ALEPH_HIP_PLANNER_UNFUSED = 0, ALEPH_HIP_PLANNER_RULE = 1, ALEPH_HIP_PLANNER_LEARNED = 2 };
+
+typedef struct aleph_hip_plan_opts {
+  uint32_t planner;         /* ALEPH_HIP_PLANNER_* */
+  uint32_t local_qubits;    /* qubits per LDS chunk (0 = default for the precision) */
+  uint32_t max_ops_per_pass;/* compute budget per pass, in 1q-gate equivalents (0 = default) */
+  uint32_t coalesce_bits;   /* low qubits kept in every chunk for coalescing (0 = default) */
+  uint32_t fuse_1q;         /* merge adjacent 1q gates on one qubit before planning */
+  uint32_t reserved[3];
+} aleph_hip_plan_opts;
+
+typedef struct aleph_hip_plan_stats {
+  double plan_ms;           /* host planning wall time */
+  uint32_t input_gates;
+  uint32_t n_passes;        /* full-state sweeps */
+  uint32_t n_stages;        /* register stages summed over passes */
+  uint32_t n_ops;           /* device ops after lowering */
+  uint32_t n_launches;
+  uint32_t reserved;
+  double sweep_bytes;       /* algorithmic HBM bytes over all passes */
+} aleph_hip_plan_stats;
+
+aleph_hip_status aleph_hip_plan_create(aleph_hip_ctx* ctx, uint32_t num_qubits, uint32_t precision,
+                                       const uint32_t* words, size_t n_words, const double* reals,
+                                       size_t n_reals, const aleph_hip_plan_opts* opts, aleph_hip_plan** out);
+aleph_hip_status aleph_hip_plan_execute(aleph_hip_ctx* ctx, aleph_hip_plan* plan, aleph_hip_state* st);
+/* Execute with a hipEvent pair around every launch; ms[i] receives the
+ * device time of launch i (n_launches entries, see aleph_hip_plan_stats),
+ * kinds[i] its kind (0 fused pass, 1 dense block, 2 diagonal, 3 phase
+ * polynomial, 4 one-qubit, 5 two-qubit, 6 swap).  This is the timing
+ * recorder the plan cost models train on. */
+aleph_hip_status aleph_hip_plan_execute_timed(aleph_hip_ctx* ctx, aleph_hip_plan* plan, aleph_hip_state* st,
+                                             double* ms, uint32_t* kinds);
+aleph_hip_status aleph_hip_plan_stats_get(const aleph_hip_plan* plan, aleph_hip_plan_stats* out);
+void aleph_hip_plan_free(aleph_hip_plan* plan);
+/* plan_create + execute + free in one call (run_with_outcomes without measurements, lib.rs:211-248). */
+aleph_hip_status aleph_hip_run_program(aleph_hip_ctx* ctx, aleph_hip_state* st, const uint32_t* words,
+                                       size_t n_words, const double* reals, size_t n_reals,
+                                       const aleph_hip_plan_opts* opts, aleph_hip_plan_stats* stats);
+
+/* ---- readout (crates/aleph-sv/src/measure.rs; crates/aleph-cuda/src/sv/readout.rs) ---- */
+/* Σ|ψ|² accumulated in FP64 with compensation (no validation). */
+aleph_hip_status aleph_hip_norm_sqr(aleph_hip_ctx* ctx, aleph_hip_state* st, double* out);
+/* validate_state: finite amplitudes and |Σ|ψ|² − 1| <= sqrt(2^n)·1e-10 (measure.rs:23-39). */
+aleph_hip_status aleph_hip_validate(aleph_hip_ctx* ctx, aleph_hip_state* st);
+aleph_hip_status aleph_hip_measure(aleph_hip_ctx* ctx, aleph_hip_state* st, uint32_t qubit, int32_t* outcome);
+aleph_hip_status aleph_hip_measure_with_uniform(aleph_hip_ctx* ctx, aleph_hip_state* st, uint32_t qubit,
+                                                double u, int32_t* outcome, int32_t* consumed);
+aleph_hip_status aleph_hip_sample(aleph_hip_ctx* ctx, aleph_hip_state* st, uint32_t shots, uint64_t* out);
+aleph_hip_status aleph_hip_sample_with_uniforms(aleph_hip_ctx* ctx, aleph_hip_state* st, const double* uniforms,
+                                                uint32_t shots, uint64_t* out);
+/* paulis[i]: 0 = I, 1 = X, 2 = Y, 3 = Z on qubits[i]. */
+aleph_hip_status aleph_hip_expectation(aleph_hip_ctx* ctx, aleph_hip_state* st, double coefficient,
+                                       const uint32_t* qubits, const uint8_t* paulis, uint32_t n_terms,
+                                       double* out);
+/* Marginal over `qubits` (qubits[0] = LSB of the output index), 2^n_qubits doubles. */
+aleph_hip_status aleph_hip_probabilities(aleph_hip_ctx* ctx, aleph_hip_state* st, const uint32_t* qubits,
+                                         uint32_t n_qubits, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ALEPH_HIP_H */

@@ -1,0 +1,5 @@
+OPENQASM 3.0;
+include "stdgates.inc";
+qubit[2] q;
+x q[0];
+cx q[0], q[1];

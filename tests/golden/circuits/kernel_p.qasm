@@ -1,0 +1,5 @@
+OPENQASM 3.0;
+include "stdgates.inc";
+qubit[1] q;
+h q[0];
+p(pi/3) q[0];

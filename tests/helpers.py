@@ -1,0 +1,97 @@
+"""Shared test helpers: fixture loading, brute-force operators, statistics."""
+
+from __future__ import annotations
+
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+STATE_TOLERANCE = 1e-10        # crates/aleph-oracle/src/harness.rs:18
+FP32_STATE_TOLERANCE = 1e-5    # harness.rs:27
+DISTRIBUTION_SHOTS = 100_000   # harness.rs:136
+DISTRIBUTION_FLOOR = 1e-6      # harness.rs:144
+PARITY_TOL = 1e-12             # GPU FP64 amplitudes vs the CPU FP64 oracle
+
+
+def load_fixture(name: str) -> dict:
+    return json.loads((GOLDEN / "aer" / f"{name}.json").read_text())
+
+
+def fixture_names() -> list[str]:
+    return sorted(p.stem for p in (GOLDEN / "aer").glob("*.json"))
+
+
+def fixture_amplitudes(fx: dict) -> np.ndarray:
+    a = np.asarray(fx["statevector"]["amplitudes"], dtype=np.float64)
+    return a[:, 0] + 1j * a[:, 1]
+
+
+def fixture_circuit(name: str):
+    from aleph_amd.workloads import load_qasm
+    return load_qasm(GOLDEN / "circuits" / f"{name}.qasm")
+
+
+def has_measure(circuit) -> bool:
+    return any(isinstance(i, tuple) and i[0] == "measure" for i in circuit.instructions)
+
+
+def embed(matrix: np.ndarray, qubits, controls, n: int) -> np.ndarray:
+    """Full 2^n x 2^n operator of a gate by enumerating basis states:
+    column x maps to sum_s' M[s', s(x)] |x with target bits replaced by s'>,
+    where s(x) reads qubits[0] as the most significant bit; identity where a
+    control bit is 0."""
+    dim = 1 << n
+    k = len(qubits)
+    U = np.zeros((dim, dim), dtype=np.complex128)
+    cmask = sum(1 << c for c in controls)
+    tmask = sum(1 << q for q in qubits)
+    for x in range(dim):
+        if (x & cmask) != cmask:
+            U[x, x] = 1.0
+            continue
+        s = 0
+        for j, q in enumerate(qubits):
+            s |= ((x >> q) & 1) << (k - 1 - j)
+        base = x & ~tmask
+        for s2 in range(1 << k):
+            y = base
+            for j, q in enumerate(qubits):
+                if (s2 >> (k - 1 - j)) & 1:
+                    y |= 1 << q
+            U[y, x] += matrix[s2, s]
+    return U
+
+
+def dense_unitary(circuit, gate_matrix) -> np.ndarray:
+    """Product of the embedded operators of every gate of a circuit."""
+    from aleph_amd.circuit import GateInstance
+    n = circuit.num_qubits
+    U = np.eye(1 << n, dtype=np.complex128)
+    for inst in circuit.instructions:
+        if isinstance(inst, GateInstance):
+            U = embed(gate_matrix(inst.gate), inst.qubits, inst.controls, n) @ U
+    return U
+
+
+def assert_distribution_close(empirical_counts: np.ndarray, exact: np.ndarray, shots: int, name: str = "") -> None:
+    """Per-outcome 5 sigma Bernoulli band + 1e-6 floor (harness.rs:225-270)."""
+    emp = empirical_counts / shots
+    for i, (e, p) in enumerate(zip(emp, exact)):
+        sigma = math.sqrt(max(p * (1 - p), 0.0) / shots)
+        band = 5 * sigma + DISTRIBUTION_FLOOR
+        assert abs(e - p) <= band, f"{name}: outcome {i} empirical {e} exact {p} band {band}"
+
+
+def random_unitary(dim: int, rng: np.random.Generator) -> np.ndarray:
+    z = (rng.standard_normal((dim, dim)) + 1j * rng.standard_normal((dim, dim))) / math.sqrt(2)
+    q, r = np.linalg.qr(z)
+    d = np.diag(r)
+    return q * (d / np.abs(d))
+
+
+def max_abs(a: np.ndarray, b: np.ndarray) -> float:
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b)))) if np.size(a) else 0.0
