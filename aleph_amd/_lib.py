@@ -94,6 +94,9 @@ SIGNATURES = {
     "aleph_hip_plan_execute_timed": (i32, [P, P, P, PF64, PU32]),
     "aleph_hip_plan_stats_get": (i32, [P, C.POINTER(PlanStats)]),
     "aleph_hip_plan_free": (None, [P]),
+    "aleph_hip_plan_build_host": (i32, [u32, u32, PU32, C.c_size_t, PF64, C.c_size_t, C.POINTER(PlanOpts),
+                                        C.POINTER(P), C.POINTER(Error)]),
+    "aleph_hip_plan_view_get": (i32, [P, P]),
     "aleph_hip_run_program": (i32, [P, P, PU32, C.c_size_t, PF64, C.c_size_t, C.POINTER(PlanOpts),
                                     C.POINTER(PlanStats)]),
     "aleph_hip_norm_sqr": (i32, [P, P, PF64]),

@@ -29,6 +29,7 @@ COMMON = [
     "-fPIC",
     f"--offload-arch={ARCH}",
     "-munsafe-fp-atomics",
+    "-ffp-contract=fast",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
     "-Wno-unused-result",

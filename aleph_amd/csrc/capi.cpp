@@ -558,12 +558,9 @@ aleph_hip_status aleph_hip_apply_swap(aleph_hip_ctx* c, aleph_hip_state* st, uin
   });
 }
 
-aleph_hip_status aleph_hip_plan_create(aleph_hip_ctx* c, uint32_t n, uint32_t precision, const uint32_t* words,
-                                       size_t n_words, const double* reals, size_t n_reals,
-                                       const aleph_hip_plan_opts* opts, aleph_hip_plan** out) {
-  return guarded(c, [&] {
-    need(c != nullptr && out != nullptr, "null argument");
-    *out = nullptr;
+namespace {
+std::unique_ptr<aleph_hip_plan> make_plan(uint32_t n, uint32_t precision, const uint32_t* words, size_t n_words,
+                                          const double* reals, size_t n_reals, const aleph_hip_plan_opts* opts) {
     need(precision == ALEPH_HIP_F64 || precision == ALEPH_HIP_F32, "precision must be 64 or 32");
     need(n >= 1 && n <= 40, "qubit count must be in [1, 40]");
     need(n_words == 0 || words != nullptr, "null words");
@@ -578,20 +575,71 @@ aleph_hip_status aleph_hip_plan_create(aleph_hip_ctx* c, uint32_t n, uint32_t pr
     }
     need(po.planner <= ALEPH_HIP_PLANNER_LEARNED, "unknown planner");
     need(po.local_qubits == 0 || (po.local_qubits >= (uint32_t)kRegBits && po.local_qubits <= (uint32_t)kMaxLocalBits),
-         "local_qubits must be in [4, 14]");
+         "local_qubits must be in [4, 13]");
     need(po.local_qubits == 0 || precision == ALEPH_HIP_F32 || po.local_qubits <= 13, "F64 chunks are at most 2^13");
     const std::vector<ProgInstr> prog = decode_program(words, n_words, reals, n_reals, n);
     auto h = std::make_unique<aleph_hip_plan>();
     h->plan = build_plan(prog, n, precision, po);
+    return h;
+}
+}  // namespace
+
+aleph_hip_status aleph_hip_plan_create(aleph_hip_ctx* c, uint32_t n, uint32_t precision, const uint32_t* words,
+                                       size_t n_words, const double* reals, size_t n_reals,
+                                       const aleph_hip_plan_opts* opts, aleph_hip_plan** out) {
+  return guarded(c, [&] {
+    need(c != nullptr && out != nullptr, "null argument");
+    *out = nullptr;
+    auto h = make_plan(n, precision, words, n_words, reals, n_reals, opts);
     h->plan->upload(c->stream);
     *out = h.release();
   });
+}
+
+aleph_hip_status aleph_hip_plan_build_host(uint32_t n, uint32_t precision, const uint32_t* words, size_t n_words,
+                                           const double* reals, size_t n_reals, const aleph_hip_plan_opts* opts,
+                                           aleph_hip_plan** out, aleph_hip_error* err) {
+  if (!out) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  try {
+    *out = make_plan(n, precision, words, n_words, reals, n_reals, opts).release();
+    return ALEPH_HIP_OK;
+  } catch (const Failure& f) {
+    if (err) {
+      aleph_hip_ctx tmp;
+      set_error(&tmp, f);
+      *err = tmp.err;
+    }
+    return f.code;
+  }
+}
+
+aleph_hip_status aleph_hip_plan_view_get(const aleph_hip_plan* plan, aleph_hip_plan_view* out) {
+  if (!plan || !plan->plan || !out) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  const Plan& p = *plan->plan;
+  out->steps = p.steps.data();
+  out->n_steps = (uint32_t)p.steps.size();
+  out->step_bytes = sizeof(Step);
+  out->passes = p.passes.data();
+  out->n_passes = (uint32_t)p.passes.size();
+  out->pass_bytes = sizeof(PassDev);
+  out->stages = p.stages.data();
+  out->n_stages = (uint32_t)p.stages.size();
+  out->stage_bytes = sizeof(StageDev);
+  out->ops = p.ops.data();
+  out->n_ops = (uint32_t)p.ops.size();
+  out->op_bytes = sizeof(DevOp);
+  out->terms = p.terms.data();
+  out->n_terms = (uint32_t)p.terms.size();
+  out->term_bytes = sizeof(PhaseTerm);
+  return ALEPH_HIP_OK;
 }
 
 aleph_hip_status aleph_hip_plan_execute(aleph_hip_ctx* c, aleph_hip_plan* plan, aleph_hip_state* st) {
   return guarded(c, [&] {
     check_state(c, st);
     need(plan != nullptr && plan->plan != nullptr, "null plan");
+    need(plan->plan->uploaded, "plan was built on the host only");
     need(plan->plan->n == st->n && plan->plan->precision == st->precision, "plan built for another state shape");
     plan->plan->execute(c->stream, st->d);
   });
@@ -604,6 +652,7 @@ aleph_hip_status aleph_hip_plan_execute_timed(aleph_hip_ctx* c, aleph_hip_plan* 
     need(plan != nullptr && plan->plan != nullptr, "null plan");
     need(ms != nullptr, "null timing output");
     const Plan& p = *plan->plan;
+    need(p.uploaded, "plan was built on the host only");
     need(p.n == st->n && p.precision == st->precision, "plan built for another state shape");
     const size_t ns = p.steps.size();
     std::vector<hipEvent_t> ev(ns + 1, nullptr);

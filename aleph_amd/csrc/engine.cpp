@@ -175,7 +175,7 @@ LGate lower_gate(const DenseGate& d) {
     if (unit && k <= 6) {
       g.kind = LGate::TERMS;
       diag_to_terms(d, cm, g.terms);
-      g.cost = 0.3 * (double)g.terms.size();
+      g.cost = 0.05 * (double)g.terms.size();
       return g;
     }
     if (k <= 2) {
@@ -281,7 +281,7 @@ std::vector<LGate> lower_program(const std::vector<ProgInstr>& prog, uint32_t n)
         for (uint64_t c : pi.conds[t]) m |= c;
         g.terms.emplace_back(m, pi.angles[t]);
       }
-      g.cost = 0.3 * (double)g.terms.size();
+      g.cost = 0.05 * (double)g.terms.size();
     } else {
       g.kind = LGate::POLY;
       g.poly_off.push_back(0);
@@ -585,6 +585,10 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
     }
     flush_phase();
     sd.op_end = (uint32_t)p.ops.size();
+    for (uint32_t o = sd.op_begin; o < sd.op_end; o++) {
+      if (p.ops[o].type == OP_2Q) pd.features |= kFeat2Q;
+      if (p.ops[o].type == OP_PHASE) pd.features |= kFeatPhase;
+    }
     p.stages.push_back(sd);
   }
   Step s;
@@ -752,6 +756,7 @@ void Plan::upload(hipStream_t s) {
   upload_vec(terms, &d_terms, s);
   upload_vec(payload, &d_payload, s);
   ALEPH_HIP_CHECK(hipStreamSynchronize(s));
+  uploaded = true;
 }
 
 template <typename T>
@@ -761,7 +766,8 @@ static void execute_t(const Plan& p, hipStream_t s, void* state, hipEvent_t* ev)
     if (ev) ALEPH_HIP_CHECK(hipEventRecord(ev[i], s));
     switch (st.kind) {
       case Step::FUSED:
-        launch_fused_pass<T>(s, state, p.n, st.m, st.n_stages, p.d_passes + st.pass, p.d_stages, p.d_ops, p.d_terms);
+        launch_fused_pass<T>(s, state, p.n, st.m, st.n_stages, p.passes[st.pass].features, p.d_passes + st.pass,
+                             p.d_stages, p.d_ops, p.d_terms);
         break;
       case Step::KQ:
         launch_kq<T>(s, state, p.n, st.k, st.qubits, st.cmask, p.d_payload + st.dev_off);

@@ -60,8 +60,10 @@ struct PlanOptions {
 };
 
 // One device launch of a plan.
+// Standard-layout so the host plan view (aleph_hip_plan_view_get) can expose it.
 struct Step {
-  enum Kind { FUSED, KQ, DIAG, POLY, G1Q, G2Q, SWAPQ } kind = FUSED;
+  enum Kind : uint32_t { FUSED, KQ, DIAG, POLY, G1Q, G2Q, SWAPQ };
+  uint32_t kind = FUSED;
   uint32_t pass = 0;          // FUSED: index into passes
   uint32_t m = 0, n_stages = 0;
   // per-gate launches
@@ -91,6 +93,7 @@ struct Plan {
   PhaseTerm* d_terms = nullptr;
   unsigned char* d_payload = nullptr;
   aleph_hip_plan_stats stats{};
+  bool uploaded = false;
   ~Plan();
   void upload(hipStream_t s);
   void execute(hipStream_t s, void* state) const;

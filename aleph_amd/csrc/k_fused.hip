@@ -190,13 +190,20 @@ __device__ __forceinline__ void op_phase(C2<T> (&a)[kRegAmps], const DevOp* __re
   }
 }
 
-template <typename T>
+template <typename T, int FEAT>
 __device__ __forceinline__ void run_ops(C2<T> (&a)[kRegAmps], const DevOp* __restrict__ ops,
                                         const PhaseTerm* __restrict__ terms, uint32_t begin, uint32_t end,
                                         uint64_t tgo, const uint64_t (&roff)[kRegAmps]) {
   for (uint32_t o = begin; o < end; o++) {
     const DevOp* op = ops + o;
-    switch (op->type) {
+    const uint32_t type = op->type;
+    if constexpr ((FEAT & kFeat2Q) == 0) {
+      if (type == OP_2Q) continue;
+    }
+    if constexpr ((FEAT & kFeatPhase) == 0) {
+      if (type == OP_PHASE) continue;
+    }
+    switch (type) {
       case OP_1Q:
         switch (op->a) {
           case 0: op_1q<T, 0>(a, op, tgo, roff); break;
@@ -206,7 +213,7 @@ __device__ __forceinline__ void run_ops(C2<T> (&a)[kRegAmps], const DevOp* __res
         }
         break;
       case OP_2Q:
-        switch (op->a * 4 + op->b) {
+        if constexpr ((FEAT & kFeat2Q) != 0) switch (op->a * 4 + op->b) {
           case 1 * 4 + 0: op_2q<T, 1, 0>(a, op, tgo, roff); break;
           case 2 * 4 + 0: op_2q<T, 2, 0>(a, op, tgo, roff); break;
           case 3 * 4 + 0: op_2q<T, 3, 0>(a, op, tgo, roff); break;
@@ -217,13 +224,15 @@ __device__ __forceinline__ void run_ops(C2<T> (&a)[kRegAmps], const DevOp* __res
         break;
       case OP_DIAG: op_diag<T>(a, op, tgo, roff); break;
       case OP_SIGN: op_sign<T>(a, op, tgo); break;
-      default: op_phase<T>(a, op, terms, tgo); break;
+      default:
+        if constexpr ((FEAT & kFeatPhase) != 0) op_phase<T>(a, op, terms, tgo);
+        break;
     }
   }
 }
 
-template <typename T, int SW, int BT>
-__global__ void __launch_bounds__(BT) k_fused_pass(C2<T>* __restrict__ state, const PassDev* __restrict__ pd,
+template <typename T, int SW, int BT, int FEAT>
+__global__ void __launch_bounds__(BT, BT == 256 ? 2 : 1) k_fused_pass(C2<T>* __restrict__ state, const PassDev* __restrict__ pd,
                                                      const StageDev* __restrict__ stages,
                                                      const DevOp* __restrict__ ops,
                                                      const PhaseTerm* __restrict__ terms) {
@@ -262,7 +271,7 @@ __global__ void __launch_bounds__(BT) k_fused_pass(C2<T>* __restrict__ state, co
 #pragma unroll
       for (int r = 0; r < kRegAmps; r++) a[r] = lds[swz<SW>(tlo | loff[r])];
     }
-    run_ops<T>(a, ops, terms, sd->op_begin, sd->op_end, tgo, roff);
+    run_ops<T, FEAT>(a, ops, terms, sd->op_begin, sd->op_end, tgo, roff);
     if (s + 1 == ns) {
 #pragma unroll
       for (int r = 0; r < kRegAmps; r++) state[tgo | roff[r]] = a[r];
@@ -277,7 +286,7 @@ __global__ void __launch_bounds__(BT) k_fused_pass(C2<T>* __restrict__ state, co
 
 }  // namespace
 
-template <typename T, int BT>
+template <typename T, int BT, int FEAT>
 static void launch_bt(hipStream_t stream, void* state, uint32_t n, uint32_t m, uint32_t n_stages, const PassDev* pd,
                       const StageDev* stages, const DevOp* ops, const PhaseTerm* terms) {
   constexpr int SW = sizeof(T) == 8 ? 3 : 4;
@@ -286,27 +295,37 @@ static void launch_bt(hipStream_t stream, void* state, uint32_t n, uint32_t m, u
   const size_t lds = n_stages > 1 ? (sizeof(C2<T>) << m) : 0;
   static bool configured = false;
   if (!configured) {
-    ALEPH_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_pass<T, SW, BT>),
+    ALEPH_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_pass<T, SW, BT, FEAT>),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     configured = true;
   }
-  hipLaunchKernelGGL((k_fused_pass<T, SW, BT>), dim3((uint32_t)blocks), dim3(threads), lds, stream,
+  hipLaunchKernelGGL((k_fused_pass<T, SW, BT, FEAT>), dim3((uint32_t)blocks), dim3(threads), lds, stream,
                      reinterpret_cast<C2<T>*>(state), pd, stages, ops, terms);
   ALEPH_HIP_CHECK(hipGetLastError());
 }
 
-template <typename T>
-void launch_fused_pass(hipStream_t stream, void* state, uint32_t n, uint32_t m, uint32_t n_stages,
-                       const PassDev* pd, const StageDev* stages, const DevOp* ops, const PhaseTerm* terms) {
+template <typename T, int FEAT>
+static void launch_feat(hipStream_t stream, void* state, uint32_t n, uint32_t m, uint32_t n_stages, const PassDev* pd,
+                        const StageDev* stages, const DevOp* ops, const PhaseTerm* terms) {
   const uint32_t threads = 1u << (m - kRegBits);
-  if (threads <= 256) launch_bt<T, 256>(stream, state, n, m, n_stages, pd, stages, ops, terms);
-  else if (threads <= 512) launch_bt<T, 512>(stream, state, n, m, n_stages, pd, stages, ops, terms);
-  else launch_bt<T, 1024>(stream, state, n, m, n_stages, pd, stages, ops, terms);
+  if (threads <= 256) launch_bt<T, 256, FEAT>(stream, state, n, m, n_stages, pd, stages, ops, terms);
+  else launch_bt<T, 512, FEAT>(stream, state, n, m, n_stages, pd, stages, ops, terms);
 }
 
-template void launch_fused_pass<double>(hipStream_t, void*, uint32_t, uint32_t, uint32_t, const PassDev*,
+template <typename T>
+void launch_fused_pass(hipStream_t stream, void* state, uint32_t n, uint32_t m, uint32_t n_stages, uint32_t feat,
+                       const PassDev* pd, const StageDev* stages, const DevOp* ops, const PhaseTerm* terms) {
+  switch (feat & (kFeat2Q | kFeatPhase)) {
+    case 0: launch_feat<T, 0>(stream, state, n, m, n_stages, pd, stages, ops, terms); break;
+    case kFeat2Q: launch_feat<T, kFeat2Q>(stream, state, n, m, n_stages, pd, stages, ops, terms); break;
+    case kFeatPhase: launch_feat<T, kFeatPhase>(stream, state, n, m, n_stages, pd, stages, ops, terms); break;
+    default: launch_feat<T, kFeat2Q | kFeatPhase>(stream, state, n, m, n_stages, pd, stages, ops, terms); break;
+  }
+}
+
+template void launch_fused_pass<double>(hipStream_t, void*, uint32_t, uint32_t, uint32_t, uint32_t, const PassDev*,
                                         const StageDev*, const DevOp*, const PhaseTerm*);
-template void launch_fused_pass<float>(hipStream_t, void*, uint32_t, uint32_t, uint32_t, const PassDev*,
+template void launch_fused_pass<float>(hipStream_t, void*, uint32_t, uint32_t, uint32_t, uint32_t, const PassDev*,
                                        const StageDev*, const DevOp*, const PhaseTerm*);
 
 }  // namespace aleph

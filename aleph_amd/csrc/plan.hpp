@@ -17,7 +17,7 @@ namespace aleph {
 
 constexpr int kRegBits = 4;                 // G: amplitudes per thread = 16
 constexpr int kRegAmps = 1 << kRegBits;
-constexpr int kMaxLocalBits = 14;           // chunk <= 2^14 amplitudes (1024 threads)
+constexpr int kMaxLocalBits = 13;           // chunk <= 2^13 amplitudes (512 threads)
 constexpr int kMaxThreadBits = kMaxLocalBits - kRegBits;
 
 enum OpType : uint32_t {
@@ -62,12 +62,17 @@ struct alignas(16) StageDev {
   uint32_t rl[kRegBits];         // chunk-local mask of register bit g
 };
 
+// Op families a pass uses; the launcher picks the kernel variant compiled
+// for exactly those, keeping register pressure of unused paths out.
+constexpr uint32_t kFeat2Q = 1;
+constexpr uint32_t kFeatPhase = 2;
+
 struct alignas(16) PassDev {
   uint32_t m;                    // chunk qubits
   uint32_t n_stages;
   uint32_t stage_begin;
-  uint32_t pad;
-  uint32_t local_pos[kMaxLocalBits];  // ascending physical positions of chunk qubits
+  uint32_t features;             // kFeat* bits
+  uint32_t local_pos[14];        // ascending physical positions of chunk qubits
   uint32_t pad2[2];
 };
 

@@ -1,0 +1,105 @@
+"""Host-side plan inspection: plan a circuit without a device and read the
+descriptor arrays (layouts of aleph_amd/csrc/plan.hpp and engine.hpp)."""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+u32, u64, f64 = C.c_uint32, C.c_uint64, C.c_double
+
+STEP_KINDS = ["fused", "kq", "diag", "poly", "1q", "2q", "swap"]
+OP_NAMES = {1: "1q", 2: "2q", 3: "diag", 4: "sign", 5: "phase"}
+
+
+class PassDev(C.Structure):
+    _fields_ = [("m", u32), ("n_stages", u32), ("stage_begin", u32), ("features", u32), ("local_pos", u32 * 14),
+                ("pad2", u32 * 2)]
+
+
+class StageDev(C.Structure):
+    _fields_ = [("op_begin", u32), ("op_end", u32), ("n_thread_bits", u32), ("pad", u32), ("tg", u64 * 10),
+                ("tl", u32 * 10), ("pad2", u32 * 2), ("rg", u64 * 4), ("rl", u32 * 4)]
+
+
+class DevOp(C.Structure):
+    _fields_ = [("type", u32), ("a", u32), ("b", u32), ("n_terms", u32), ("cmask", u64), ("ma", u64), ("mb", u64),
+                ("term_off", u32), ("pad", u32), ("m", f64 * 32)]
+
+
+class PhaseTerm(C.Structure):
+    _fields_ = [("mask_out", u64), ("angle", f64), ("cls", u32), ("rmask", u32), ("pad", u32 * 2)]
+
+
+class Step(C.Structure):
+    _fields_ = [("kind", u32), ("pass_", u32), ("m", u32), ("n_stages", u32), ("k", u32), ("qubits", u32 * 8),
+                ("cmask", u64), ("mat8", f64 * 32), ("dev_off", C.c_size_t), ("dev_off2", C.c_size_t),
+                ("dev_off3", C.c_size_t), ("count", u32)]
+
+
+class PlanView(C.Structure):
+    _fields_ = [("steps", C.c_void_p), ("n_steps", u32), ("step_bytes", u32),
+                ("passes", C.c_void_p), ("n_passes", u32), ("pass_bytes", u32),
+                ("stages", C.c_void_p), ("n_stages", u32), ("stage_bytes", u32),
+                ("ops", C.c_void_p), ("n_ops", u32), ("op_bytes", u32),
+                ("terms", C.c_void_p), ("n_terms", u32), ("term_bytes", u32)]
+
+
+def _array(ptr, count, cls, nbytes):
+    if C.sizeof(cls) != nbytes:
+        raise RuntimeError(f"{cls.__name__} layout mismatch: {C.sizeof(cls)} != {nbytes}")
+    if count == 0:
+        return []
+    arr = (cls * count).from_address(ptr)
+    return [arr[i] for i in range(count)]
+
+
+class HostPlan:
+    """A plan built on the host; arrays are copied out on construction."""
+
+    def __init__(self, circuit, precision: int = 64, planner: int = L.PLANNER_RULE, local_qubits: int = 0,
+                 max_ops: int = 0, coalesce_bits: int = 0, fuse_1q: bool = True):
+        lib = L.load()
+        words, reals = circuit.encode()
+        nw, nr = int(words.size), int(reals.size)
+        words = np.ascontiguousarray(words if nw else np.zeros(1, np.uint32))
+        reals = np.ascontiguousarray(reals if nr else np.zeros(1))
+        opts = L.PlanOpts(planner, local_qubits, max_ops, coalesce_bits, int(fuse_1q))
+        h = C.c_void_p()
+        err = L.Error()
+        code = lib.aleph_hip_plan_build_host(circuit.num_qubits, precision, words.ctypes.data_as(L.PU32), nw,
+                                             reals.ctypes.data_as(L.PF64), nr, C.byref(opts), C.byref(h),
+                                             C.byref(err))
+        if code != L.OK:
+            raise RuntimeError(f"plan failed with status {code}: {err.message.decode()}")
+        try:
+            v = PlanView()
+            lib.aleph_hip_plan_view_get(h, C.byref(v))
+            self.steps = [Step.from_buffer_copy(s) for s in _array(v.steps, v.n_steps, Step, v.step_bytes)]
+            self.passes = [PassDev.from_buffer_copy(s) for s in _array(v.passes, v.n_passes, PassDev, v.pass_bytes)]
+            self.stages = [StageDev.from_buffer_copy(s) for s in _array(v.stages, v.n_stages, StageDev,
+                                                                         v.stage_bytes)]
+            self.ops = [DevOp.from_buffer_copy(s) for s in _array(v.ops, v.n_ops, DevOp, v.op_bytes)]
+            self.terms = [PhaseTerm.from_buffer_copy(s) for s in _array(v.terms, v.n_terms, PhaseTerm,
+                                                                         v.term_bytes)]
+            st = L.PlanStats()
+            lib.aleph_hip_plan_stats_get(h, C.byref(st))
+            self.stats = {f: getattr(st, f) for f, _ in L.PlanStats._fields_ if f != "reserved"}
+        finally:
+            lib.aleph_hip_plan_free(h)
+        self.num_qubits = circuit.num_qubits
+
+    def summary(self) -> list[dict]:
+        out = []
+        for s in self.steps:
+            d = {"kind": STEP_KINDS[s.kind]}
+            if s.kind == 0:
+                p = self.passes[s.pass_]
+                stages = self.stages[p.stage_begin:p.stage_begin + p.n_stages]
+                d.update(m=p.m, chunk=list(p.local_pos[:p.m]), stages=p.n_stages,
+                         ops=[[OP_NAMES[self.ops[o].type] for o in range(sd.op_begin, sd.op_end)] for sd in stages])
+            out.append(d)
+        return out

@@ -204,6 +204,22 @@ aleph_hip_status aleph_hip_plan_execute_timed(aleph_hip_ctx* ctx, aleph_hip_plan
                                              double* ms, uint32_t* kinds);
 aleph_hip_status aleph_hip_plan_stats_get(const aleph_hip_plan* plan, aleph_hip_plan_stats* out);
 void aleph_hip_plan_free(aleph_hip_plan* plan);
+/* Diagnostics: plan on the host only (no device needed; such a plan cannot
+ * be executed) and read the plan's descriptor arrays, whose layouts are
+ * those of aleph_amd/csrc/plan.hpp (PassDev, StageDev, DevOp, PhaseTerm)
+ * and engine.hpp (Step).  Used by planner analysis and host-side tests. */
+typedef struct aleph_hip_plan_view {
+  const void* steps; uint32_t n_steps; uint32_t step_bytes;
+  const void* passes; uint32_t n_passes; uint32_t pass_bytes;
+  const void* stages; uint32_t n_stages; uint32_t stage_bytes;
+  const void* ops; uint32_t n_ops; uint32_t op_bytes;
+  const void* terms; uint32_t n_terms; uint32_t term_bytes;
+} aleph_hip_plan_view;
+aleph_hip_status aleph_hip_plan_build_host(uint32_t num_qubits, uint32_t precision, const uint32_t* words,
+                                           size_t n_words, const double* reals, size_t n_reals,
+                                           const aleph_hip_plan_opts* opts, aleph_hip_plan** out,
+                                           aleph_hip_error* err);
+aleph_hip_status aleph_hip_plan_view_get(const aleph_hip_plan* plan, aleph_hip_plan_view* out);
 /* plan_create + execute + free in one call (run_with_outcomes without measurements, lib.rs:211-248). */
 aleph_hip_status aleph_hip_run_program(aleph_hip_ctx* ctx, aleph_hip_state* st, const uint32_t* words,
                                        size_t n_words, const double* reals, size_t n_reals,
