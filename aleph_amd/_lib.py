@@ -39,6 +39,8 @@ ERR_OUT_OF_MEMORY = 18
 
 F64, F32 = 64, 32
 PLANNER_UNFUSED, PLANNER_RULE, PLANNER_LEARNED = 0, 1, 2
+PLAN_NONTEMPORAL = 1
+PLAN_NO_FUSE_2Q = 2
 INSTR_DIAGONAL_PHASE, INSTR_BARRIER = 64, 65
 
 
@@ -54,7 +56,7 @@ class Gate(C.Structure):
 
 class PlanOpts(C.Structure):
     _fields_ = [("planner", u32), ("local_qubits", u32), ("max_ops_per_pass", u32), ("coalesce_bits", u32),
-                ("fuse_1q", u32), ("reserved", u32 * 3)]
+                ("fuse_1q", u32), ("reg_bits", u32), ("flags", u32), ("reserved", u32)]
 
 
 class PlanStats(C.Structure):

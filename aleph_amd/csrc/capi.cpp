@@ -572,11 +572,15 @@ std::unique_ptr<aleph_hip_plan> make_plan(uint32_t n, uint32_t precision, const 
       po.max_ops = opts->max_ops_per_pass;
       po.coalesce_bits = opts->coalesce_bits;
       po.fuse_1q = opts->fuse_1q != 0;
+      po.reg_bits = opts->reg_bits;
+      po.nontemporal = (opts->flags & ALEPH_HIP_PLAN_NONTEMPORAL) != 0;
+      po.fuse_2q = (opts->flags & ALEPH_HIP_PLAN_NO_FUSE_2Q) == 0;
     }
+    need(po.reg_bits == 0 || po.reg_bits == 4 || po.reg_bits == 5, "reg_bits must be 4 or 5");
     need(po.planner <= ALEPH_HIP_PLANNER_LEARNED, "unknown planner");
-    need(po.local_qubits == 0 || (po.local_qubits >= (uint32_t)kRegBits && po.local_qubits <= (uint32_t)kMaxLocalBits),
-         "local_qubits must be in [4, 13]");
-    need(po.local_qubits == 0 || precision == ALEPH_HIP_F32 || po.local_qubits <= 13, "F64 chunks are at most 2^13");
+    need(po.local_qubits == 0 || (po.local_qubits >= (po.reg_bits ? po.reg_bits : (uint32_t)kRegBits) &&
+                                  po.local_qubits <= (uint32_t)kMaxLocalBits),
+         "local_qubits must be in [reg_bits, 13]");
     const std::vector<ProgInstr> prog = decode_program(words, n_words, reals, n_reals, n);
     auto h = std::make_unique<aleph_hip_plan>();
     h->plan = build_plan(prog, n, precision, po);
@@ -632,6 +636,8 @@ aleph_hip_status aleph_hip_plan_view_get(const aleph_hip_plan* plan, aleph_hip_p
   out->terms = p.terms.data();
   out->n_terms = (uint32_t)p.terms.size();
   out->term_bytes = sizeof(PhaseTerm);
+  out->payload = p.payload.data();
+  out->payload_bytes = p.payload.size();
   return ALEPH_HIP_OK;
 }
 

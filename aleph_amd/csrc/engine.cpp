@@ -235,7 +235,7 @@ LGate lower_gate(const DenseGate& d) {
         g.m[2 * e] = d.m[e].real();
         g.m[2 * e + 1] = d.m[e].imag();
       }
-      g.cost = dim == 2 ? 1.0 : 4.0;
+      g.cost = dim == 2 ? 1.0 : 2.0;
       return g;
     }
   }
@@ -260,7 +260,18 @@ std::vector<LGate> lower_program(const std::vector<ProgInstr>& prog, uint32_t n)
   for (const ProgInstr& pi : prog) {
     if (pi.type == ProgInstr::BARRIER) continue;
     if (pi.type == ProgInstr::GATE) {
-      out.push_back(lower_gate(pi.gate));
+      LGate g = lower_gate(pi.gate);
+      const DenseGate& d = pi.gate;
+      if (d.controls.empty() && (d.dim == 2 || d.dim == 4)) {
+        g.dense_k = d.dim == 2 ? 1 : 2;
+        g.dense_q[0] = d.qubits[0];
+        g.dense_q[1] = d.dim == 4 ? d.qubits[1] : 0;
+        for (uint32_t e = 0; e < d.dim * d.dim; e++) {
+          g.dense[2 * e] = d.m[e].real();
+          g.dense[2 * e + 1] = d.m[e].imag();
+        }
+      }
+      out.push_back(std::move(g));
       continue;
     }
     // DiagonalPhase: single-bit parity conditions are AND masks.
@@ -325,6 +336,7 @@ std::vector<LGate> fuse_1q_runs(const std::vector<LGate>& in) {
             r[2 * (i * 2 + j) + 1] = im;
           }
         std::memcpy(acc.m, r, sizeof(r));
+        std::memcpy(acc.dense, r, sizeof(r));
         continue;
       }
       open[q] = (int64_t)out.size();
@@ -338,8 +350,140 @@ std::vector<LGate> fuse_1q_runs(const std::vector<LGate>& in) {
   return out;
 }
 
+// 4x4 complex helpers (interleaved, row-major, index bit 1 = MSB qubit).
+void mul4(const double* a, const double* b, double* out) {
+  double t[32];
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) {
+      double re = 0, im = 0;
+      for (int k = 0; k < 4; k++) {
+        const double ar = a[2 * (i * 4 + k)], ai = a[2 * (i * 4 + k) + 1];
+        const double br = b[2 * (k * 4 + j)], bi = b[2 * (k * 4 + j) + 1];
+        re += ar * br - ai * bi;
+        im += ar * bi + ai * br;
+      }
+      t[2 * (i * 4 + j)] = re;
+      t[2 * (i * 4 + j) + 1] = im;
+    }
+  std::memcpy(out, t, sizeof(t));
+}
+
+// Lift a 2x2 on qubit q into the 4x4 of the pair (msb, lsb).
+void lift2(const double* m2, bool on_msb, double* out) {
+  std::memset(out, 0, 32 * sizeof(double));
+  for (int r = 0; r < 4; r++)
+    for (int c = 0; c < 4; c++) {
+      const int hr = r >> 1, lr = r & 1, hc = c >> 1, lc = c & 1;
+      int e = -1;
+      if (on_msb && lr == lc) e = hr * 2 + hc;
+      if (!on_msb && hr == hc) e = lr * 2 + lc;
+      if (e >= 0) {
+        out[2 * (r * 4 + c)] = m2[2 * e];
+        out[2 * (r * 4 + c) + 1] = m2[2 * e + 1];
+      }
+    }
+}
+
+// Pair fusion: a 2-qubit gate with at least one pending non-diagonal
+// 1-qubit gate on its qubits absorbs every pending 1-qubit gate before it
+// and every 1-qubit gate after it on either qubit up to that qubit's next
+// other gate, into one dense 4x4.  2-qubit gates without pending neighbours
+// stay as they are (a bare CZ stays a sign flip).  Moving the absorbed
+// gates to the 2-qubit gate's position is valid because no other gate
+// touches their qubit in between.
+std::vector<LGate> fuse_pairs(const std::vector<LGate>& in) {
+  const size_t N = in.size();
+  std::vector<int64_t> owner(N, -1);
+  std::vector<char> root(N, 0);
+  std::vector<std::vector<size_t>> pre(N), post(N);
+  std::vector<std::vector<size_t>> pending(64);
+  auto touches = [](const LGate& g, uint32_t q) {
+    for (uint32_t x : g.nd)
+      if (x == q) return true;
+    for (uint32_t x : g.dg)
+      if (x == q) return true;
+    return false;
+  };
+  for (size_t i = 0; i < N; i++) {
+    if (owner[i] >= 0) continue;
+    const LGate& g = in[i];
+    if (g.dense_k == 1) {
+      pending[g.dense_q[0]].push_back(i);
+      continue;
+    }
+    if (g.dense_k == 2 && (g.kind == LGate::G2Q || g.kind == LGate::SIGN || g.kind == LGate::G1Q)) {
+      const uint32_t a = g.dense_q[0], b = g.dense_q[1];
+      bool any_nd = false;
+      for (uint32_t q : {a, b})
+        for (size_t j : pending[q]) any_nd |= in[j].kind == LGate::G1Q;
+      if (any_nd) {
+        root[i] = 1;
+        for (uint32_t q : {a, b}) {
+          for (size_t j : pending[q]) {
+            owner[j] = (int64_t)i;
+            pre[i].push_back(j);
+          }
+          pending[q].clear();
+        }
+        std::sort(pre[i].begin(), pre[i].end());
+        bool open[2] = {true, true};
+        const uint32_t qs[2] = {a, b};
+        for (size_t j = i + 1; j < N && (open[0] || open[1]); j++) {
+          if (owner[j] >= 0) continue;
+          const LGate& h = in[j];
+          for (int side = 0; side < 2; side++) {
+            if (!open[side] || !touches(h, qs[side])) continue;
+            if (h.dense_k == 1 && h.dense_q[0] == qs[side]) {
+              owner[j] = (int64_t)i;
+              post[i].push_back(j);
+            } else {
+              open[side] = false;
+            }
+          }
+        }
+        continue;
+      }
+    }
+    for (uint32_t q : g.nd) pending[q].clear();
+    for (uint32_t q : g.dg) pending[q].clear();
+  }
+  std::vector<LGate> out;
+  for (size_t i = 0; i < N; i++) {
+    if (owner[i] >= 0) continue;
+    if (!root[i]) {
+      out.push_back(in[i]);
+      continue;
+    }
+    const LGate& g = in[i];
+    const uint32_t a = g.dense_q[0], b = g.dense_q[1];
+    double M[32], L[32];
+    std::memset(M, 0, sizeof(M));
+    for (int d = 0; d < 4; d++) M[2 * (d * 4 + d)] = 1.0;
+    for (size_t j : pre[i]) {
+      lift2(in[j].dense, in[j].dense_q[0] == a, L);
+      mul4(L, M, M);
+    }
+    mul4(g.dense, M, M);
+    for (size_t j : post[i]) {
+      lift2(in[j].dense, in[j].dense_q[0] == a, L);
+      mul4(L, M, M);
+    }
+    LGate f;
+    f.kind = LGate::G2Q;
+    f.qa = a;
+    f.qb = b;
+    f.nd = {a, b};
+    std::memcpy(f.m, M, sizeof(M));
+    f.cost = 2.0;
+    out.push_back(std::move(f));
+  }
+  return out;
+}
+
 struct PlannerCtx {
   uint32_t n, m, L;
+  uint32_t G;        // register bits per thread (amplitudes per thread = 2^G)
+  uint32_t feat;     // extra kFeat* bits for every pass (non-temporal streaming)
   double budget;
 };
 
@@ -357,16 +501,19 @@ struct Blocker {
   }
 };
 
-uint32_t pair_class(uint32_t rm) {
-  switch (rm) {
-    case 0x3: return 5;
-    case 0x5: return 6;
-    case 0x9: return 7;
-    case 0x6: return 8;
-    case 0xA: return 9;
-    case 0xC: return 10;
-    default: return 11;
+// PhaseSlots accumulator of a term whose register part is `rm` (G bits).
+uint32_t phase_class(uint32_t rm, uint32_t G, bool* general) {
+  *general = false;
+  const int c = popc(rm);
+  if (c == 0) return 0;
+  const uint32_t g = (uint32_t)__builtin_ctz(rm);
+  if (c == 1) return 1 + g;
+  if (c == 2) {
+    const uint32_t h = 31u - (uint32_t)__builtin_clz(rm);
+    return 1 + G + g * G - g * (g + 1) / 2 + (h - g - 1);
   }
+  *general = true;
+  return 0;
 }
 
 void swap_2q_matrix(double* m) {
@@ -449,7 +596,7 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
       for (const LGate* g : rem) {
         const uint64_t ndm = qmask(g->nd), dgm = qmask(g->dg);
         const uint32_t ndl = local_mask(ndm);
-        bool ok = bl.admits(ndm, dgm) && popc(G | ndl) <= kRegBits;
+        bool ok = bl.admits(ndm, dgm) && popc(G | ndl) <= (int)pc.G;
         if (ok && avoid_low && (ndl & low_local)) ok = false;
         if (ok) {
           take.push_back(g);
@@ -465,8 +612,8 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
     bool has_dense = false;
     for (const LGate* g : take) has_dense |= !g->nd.empty();
     if (first && pc.L > 0 && !has_dense) std::tie(G, take, left) = form(false);
-    // pad the register set to kRegBits local bits, preferring high local bits
-    for (int b = (int)m - 1; b >= 0 && popc(G) < kRegBits; b--)
+    // pad the register set to G local bits, preferring high local bits
+    for (int b = (int)m - 1; b >= 0 && popc(G) < (int)pc.G; b--)
       if (!(G & (1u << b))) G |= 1u << b;
     stage_list.emplace_back(G, take);
     rem = left;
@@ -474,7 +621,7 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
   }
   if (stage_list.empty()) stage_list.emplace_back(0u, std::vector<const LGate*>{});
   // Re-layout stages so that HBM stages keep the low chunk bits as thread bits.
-  const uint32_t top = ((1u << m) - 1u) & ~((1u << (m - kRegBits)) - 1u);
+  const uint32_t top = ((1u << m) - 1u) & ~((1u << (m - pc.G)) - 1u);
   if (pc.L > 0 && (stage_list.front().first & low_local))
     stage_list.insert(stage_list.begin(), {top, {}});
   if (pc.L > 0 && (stage_list.back().first & low_local)) stage_list.push_back({top, {}});
@@ -484,17 +631,19 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
   pd.n_stages = (uint32_t)stage_list.size();
   pd.stage_begin = (uint32_t)p.stages.size();
   for (uint32_t i = 0; i < m; i++) pd.local_pos[i] = pos[i];
+  pd.reg_bits = pc.G;
+  pd.features = pc.feat;
 
   for (auto& [G, sgates] : stage_list) {
     StageDev sd{};
-    uint32_t reg_local[kRegBits];
+    uint32_t reg_local[kMaxRegBits];
     int rcount = 0;
     for (uint32_t b = 0; b < m; b++)
       if (G & (1u << b)) reg_local[rcount++] = b;
     uint32_t reg_of_local[32];
     for (int i = 0; i < 32; i++) reg_of_local[i] = UINT32_MAX;
     uint64_t rphys = 0;
-    for (int g = 0; g < kRegBits; g++) {
+    for (int g = 0; g < (int)pc.G; g++) {
       reg_of_local[reg_local[g]] = (uint32_t)g;
       sd.rl[g] = 1u << reg_local[g];
       sd.rg[g] = 1ull << pos[reg_local[g]];
@@ -511,7 +660,7 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
     sd.op_begin = (uint32_t)p.ops.size();
     auto reg_mask = [&](uint64_t phys) {
       uint32_t rm = 0;
-      for (int g = 0; g < kRegBits; g++)
+      for (int g = 0; g < (int)pc.G; g++)
         if (phys & sd.rg[g]) rm |= 1u << g;
       return rm;
     };
@@ -574,9 +723,9 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
             t.mask_out = mask & ~rphys;
             t.rmask = reg_mask(mask);
             t.angle = angle;
-            const int pc2 = popc(t.rmask);
-            t.cls = pc2 == 0 ? 0u : pc2 == 1 ? 1u + (uint32_t)__builtin_ctz(t.rmask) : pc2 == 2 ? pair_class(t.rmask) : 11u;
-            (t.cls == 11u ? pend_gen : pend_reg).push_back(t);
+            bool general = false;
+            t.cls = phase_class(t.rmask, pc.G, &general);
+            (general ? pend_gen : pend_reg).push_back(t);
           }
           break;
         default:
@@ -606,7 +755,9 @@ void plan_rule(Plan& p, std::vector<LGate> lg, uint32_t n, const PlanOptions& o)
   const uint32_t mdef = p.precision == ALEPH_HIP_F64 ? 12u : 13u;
   pc.m = std::min<uint32_t>(o.local_qubits ? o.local_qubits : mdef, n);
   pc.m = std::min<uint32_t>(pc.m, kMaxLocalBits);
-  pc.L = std::min<uint32_t>(o.coalesce_bits ? o.coalesce_bits : 3u, pc.m - kRegBits);
+  pc.G = o.reg_bits ? o.reg_bits : (uint32_t)kRegBits;
+  pc.feat = o.nontemporal ? kFeatNontemporal : 0u;
+  pc.L = std::min<uint32_t>(o.coalesce_bits ? o.coalesce_bits : 3u, pc.m - pc.G);
   pc.budget = o.max_ops ? (double)o.max_ops : 24.0;
   const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
   std::vector<char> done(lg.size(), 0);
@@ -712,11 +863,13 @@ std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog, uint32_t n,
   p->opts = opts;
   uint32_t gates = 0;
   for (const ProgInstr& pi : prog) gates += pi.type == ProgInstr::GATE ? 1 : 0;
-  if (opts.planner == ALEPH_HIP_PLANNER_UNFUSED || n < (uint32_t)kRegBits) {
+  const uint32_t G = opts.reg_bits ? opts.reg_bits : (uint32_t)kRegBits;
+  if (opts.planner == ALEPH_HIP_PLANNER_UNFUSED || n < G) {
     plan_unfused(*p, prog);
   } else {
     std::vector<LGate> lg = lower_program(prog, n);
     if (opts.fuse_1q) lg = fuse_1q_runs(lg);
+    if (opts.fuse_2q) lg = fuse_pairs(lg);
     plan_rule(*p, std::move(lg), n, opts);
   }
   const auto t1 = std::chrono::steady_clock::now();
@@ -766,8 +919,8 @@ static void execute_t(const Plan& p, hipStream_t s, void* state, hipEvent_t* ev)
     if (ev) ALEPH_HIP_CHECK(hipEventRecord(ev[i], s));
     switch (st.kind) {
       case Step::FUSED:
-        launch_fused_pass<T>(s, state, p.n, st.m, st.n_stages, p.passes[st.pass].features, p.d_passes + st.pass,
-                             p.d_stages, p.d_ops, p.d_terms);
+        launch_fused_pass<T>(s, state, p.n, st.m, st.n_stages, p.passes[st.pass].features,
+                             p.passes[st.pass].reg_bits, p.d_passes + st.pass, p.d_stages, p.d_ops, p.d_terms);
         break;
       case Step::KQ:
         launch_kq<T>(s, state, p.n, st.k, st.qubits, st.cmask, p.d_payload + st.dev_off);

@@ -46,6 +46,12 @@ struct LGate {
   std::vector<uint64_t> poly_conds;
   std::vector<uint32_t> poly_off;
   double cost = 0.0;
+  // The original uncontrolled 1- or 2-qubit unitary (dense_k = 1 or 2,
+  // qubits MSB first), kept so neighbouring gates can be fused into one
+  // 4x4 block; dense_k = 0 when the gate is controlled or wider.
+  uint32_t dense_k = 0;
+  uint32_t dense_q[2] = {0, 0};
+  double dense[32] = {0};
   bool standalone() const { return kind == KQ || kind == POLY; }
 };
 
@@ -57,6 +63,9 @@ struct PlanOptions {
   uint32_t max_ops = 0;
   uint32_t coalesce_bits = 0;
   bool fuse_1q = true;
+  uint32_t reg_bits = 0;      // 0: default (4)
+  bool nontemporal = false;
+  bool fuse_2q = true;        // fuse 1q neighbours into 2q gates (4x4 blocks)
 };
 
 // One device launch of a plan.

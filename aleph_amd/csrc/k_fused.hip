@@ -2,11 +2,13 @@
 //
 // One workgroup = one chunk of 2^m amplitudes (the index bits outside the
 // pass's chunk qubits are fixed by blockIdx).  The chunk is read from HBM
-// once, every op of the pass is applied while it sits in registers (16
+// once, every op of the pass is applied while it sits in registers (2^G
 // amplitudes per thread) or is re-distributed through LDS between stages,
 // and it is written back once: one HBM round trip for the whole pass,
 // instead of one per gate as in the reference's per-gate launches
 // (crates/aleph-cuda/src/sv/backend.rs:515-581).
+#include <type_traits>
+
 #include "common.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
@@ -31,15 +33,34 @@ __device__ __forceinline__ C2<T> cmac2(T ar, T ai, C2<T> x, T br, T bi, C2<T> y)
   return o;
 }
 
-template <typename T, int B>
-__device__ __forceinline__ void op_1q(C2<T> (&a)[kRegAmps], const DevOp* __restrict__ op, uint64_t tgo,
-                                      const uint64_t (&roff)[kRegAmps]) {
+// Uniform global offset of register slot r (bit g of r selects rg[g]).
+template <int G>
+__device__ __forceinline__ uint64_t roff(const uint64_t (&rg)[G], int r) {
+  uint64_t o = 0;
+#pragma unroll
+  for (int g = 0; g < G; g++)
+    if (r & (1 << g)) o |= rg[g];
+  return o;
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t loff(const uint32_t (&rl)[G], int r) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int g = 0; g < G; g++)
+    if (r & (1 << g)) o |= rl[g];
+  return o;
+}
+
+template <typename T, int G, int B>
+__device__ __forceinline__ void op_1q(C2<T> (&a)[1 << G], const DevOp* __restrict__ op, uint64_t tgo,
+                                      const uint64_t (&rg)[G]) {
   const T m00r = (T)op->m[0], m00i = (T)op->m[1], m01r = (T)op->m[2], m01i = (T)op->m[3];
   const T m10r = (T)op->m[4], m10i = (T)op->m[5], m11r = (T)op->m[6], m11i = (T)op->m[7];
   const uint64_t cm = op->cmask;
   if (cm == 0) {
 #pragma unroll
-    for (int r = 0; r < kRegAmps; r++) {
+    for (int r = 0; r < (1 << G); r++) {
       if (r & (1 << B)) continue;
       const int s = r | (1 << B);
       const C2<T> x = a[r], y = a[s];
@@ -47,12 +68,13 @@ __device__ __forceinline__ void op_1q(C2<T> (&a)[kRegAmps], const DevOp* __restr
       a[s] = cmac2(m10r, m10i, x, m11r, m11i, y);
     }
   } else {
+    const uint64_t tc = tgo & cm;
 #pragma unroll
-    for (int r = 0; r < kRegAmps; r++) {
+    for (int r = 0; r < (1 << G); r++) {
       if (r & (1 << B)) continue;
       const int s = r | (1 << B);
       const C2<T> x = a[r], y = a[s];
-      const bool on = ((tgo | roff[r]) & cm) == cm;
+      const bool on = (tc | (roff<G>(rg, r) & cm)) == cm;
       const C2<T> nx = cmac2(m00r, m00i, x, m01r, m01i, y);
       const C2<T> ny = cmac2(m10r, m10i, x, m11r, m11i, y);
       a[r] = on ? nx : x;
@@ -61,54 +83,75 @@ __device__ __forceinline__ void op_1q(C2<T> (&a)[kRegAmps], const DevOp* __restr
   }
 }
 
-template <typename T, int A, int B>  // A = matrix MSB register bit, B = LSB, A > B
-__device__ __forceinline__ void op_2q(C2<T> (&a)[kRegAmps], const DevOp* __restrict__ op, uint64_t tgo,
-                                      const uint64_t (&roff)[kRegAmps]) {
-  T mr[16], mi[16];
-#pragma unroll
-  for (int e = 0; e < 16; e++) {
-    mr[e] = (T)op->m[2 * e];
-    mi[e] = (T)op->m[2 * e + 1];
-  }
-  const uint64_t cm = op->cmask;
-#pragma unroll
-  for (int r = 0; r < kRegAmps; r++) {
-    if (r & ((1 << A) | (1 << B))) continue;
-    const int idx[4] = {r, r | (1 << B), r | (1 << A), r | (1 << A) | (1 << B)};
-    C2<T> v[4], o[4];
-#pragma unroll
-    for (int c = 0; c < 4; c++) v[c] = a[idx[c]];
+// Insert a zero bit at position p of x (compile time).
+constexpr int ins0(int x, int p) { return (x & ((1 << p) - 1)) | ((x >> p) << (p + 1)); }
+
+// One quartet (register bits A > B cleared in slot Q): its four inputs are
+// copied out, then each output row is formed from one matrix row read just
+// before use.  The empty asm with a memory clobber keeps the compiler from
+// hoisting all sixteen matrix entries into registers at once.
+template <typename T, int G, int A, int B, int Q>
+__device__ __forceinline__ void quartets_2q(C2<T> (&a)[1 << G], const DevOp* __restrict__ op,
+                                            const bool (&on)[1 << (G - 2)]) {
+  if constexpr (Q < (1 << (G - 2))) {
+    constexpr int r = ins0(ins0(Q, B), A);
+    constexpr int idx[4] = {r, r | (1 << B), r | (1 << A), r | (1 << A) | (1 << B)};
+    const C2<T> v0 = a[idx[0]], v1 = a[idx[1]], v2 = a[idx[2]], v3 = a[idx[3]];
+    C2<T> out[4];
 #pragma unroll
     for (int row = 0; row < 4; row++) {
-      T re = 0, im = 0;
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        re += mr[row * 4 + c] * v[c].x - mi[row * 4 + c] * v[c].y;
-        im += mr[row * 4 + c] * v[c].y + mi[row * 4 + c] * v[c].x;
-      }
-      o[row].x = re;
-      o[row].y = im;
+      asm volatile("" ::: "memory");
+      const double* mrow = op->m + 8 * row;
+      const T m0r = (T)mrow[0], m0i = (T)mrow[1], m1r = (T)mrow[2], m1i = (T)mrow[3];
+      const T m2r = (T)mrow[4], m2i = (T)mrow[5], m3r = (T)mrow[6], m3i = (T)mrow[7];
+      T re = m0r * v0.x;
+      T im = m0r * v0.y;
+      re = fma(-m0i, v0.y, re);
+      im = fma(m0i, v0.x, im);
+      re = fma(m1r, v1.x, re);
+      re = fma(-m1i, v1.y, re);
+      im = fma(m1r, v1.y, im);
+      im = fma(m1i, v1.x, im);
+      re = fma(m2r, v2.x, re);
+      re = fma(-m2i, v2.y, re);
+      im = fma(m2r, v2.y, im);
+      im = fma(m2i, v2.x, im);
+      re = fma(m3r, v3.x, re);
+      re = fma(-m3i, v3.y, re);
+      im = fma(m3r, v3.y, im);
+      im = fma(m3i, v3.x, im);
+      out[row].x = re;
+      out[row].y = im;
     }
-    if (cm == 0) {
-#pragma unroll
-      for (int c = 0; c < 4; c++) a[idx[c]] = o[c];
-    } else {
-      const bool on = ((tgo | roff[r]) & cm) == cm;
-#pragma unroll
-      for (int c = 0; c < 4; c++) a[idx[c]] = on ? o[c] : v[c];
-    }
+    const bool live = on[Q];
+    a[idx[0]] = live ? out[0] : v0;
+    a[idx[1]] = live ? out[1] : v1;
+    a[idx[2]] = live ? out[2] : v2;
+    a[idx[3]] = live ? out[3] : v3;
+    quartets_2q<T, G, A, B, Q + 1>(a, op, on);
   }
 }
 
-template <typename T>
-__device__ __forceinline__ void op_diag(C2<T> (&a)[kRegAmps], const DevOp* __restrict__ op, uint64_t tgo,
-                                        const uint64_t (&roff)[kRegAmps]) {
+template <typename T, int G, int A, int B>  // A = matrix MSB register bit, B = LSB, A > B
+__device__ __forceinline__ void op_2q(C2<T> (&a)[1 << G], const DevOp* __restrict__ op, uint64_t tgo,
+                                      const uint64_t (&rg)[G]) {
+  const uint64_t cm = op->cmask;
+  const uint64_t tc = tgo & cm;
+  bool on[1 << (G - 2)];
+#pragma unroll
+  for (int q = 0; q < (1 << (G - 2)); q++) on[q] = (tc | (roff<G>(rg, ins0(ins0(q, B), A)) & cm)) == cm;
+  quartets_2q<T, G, A, B, 0>(a, op, on);
+}
+
+template <typename T, int G>
+__device__ __forceinline__ void op_diag(C2<T> (&a)[1 << G], const DevOp* __restrict__ op, uint64_t tgo,
+                                        const uint64_t (&rg)[G]) {
   const uint64_t ma = op->ma, mb = op->mb, cm = op->cmask;
   const T d0r = (T)op->m[0], d0i = (T)op->m[1], d1r = (T)op->m[2], d1i = (T)op->m[3];
   const T d2r = (T)op->m[4], d2i = (T)op->m[5], d3r = (T)op->m[6], d3i = (T)op->m[7];
 #pragma unroll
-  for (int r = 0; r < kRegAmps; r++) {
-    const uint64_t idx = tgo | roff[r];
+  for (int r = 0; r < (1 << G); r++) {
+    const uint64_t idx = tgo | roff<G>(rg, r);
     const bool hi = (idx & ma) != 0, lo = (idx & mb) != 0;
     const T dr = hi ? (lo ? d3r : d2r) : (lo ? d1r : d0r);
     const T di = hi ? (lo ? d3i : d2i) : (lo ? d1i : d0i);
@@ -120,62 +163,55 @@ __device__ __forceinline__ void op_diag(C2<T> (&a)[kRegAmps], const DevOp* __res
   }
 }
 
-template <typename T>
-__device__ __forceinline__ void op_sign(C2<T> (&a)[kRegAmps], const DevOp* __restrict__ op, uint64_t tgo) {
+template <typename T, int G>
+__device__ __forceinline__ void op_sign(C2<T> (&a)[1 << G], const DevOp* __restrict__ op, uint64_t tgo) {
   const uint64_t mo = op->cmask;
   const uint32_t rm = op->a;
   const bool th = (tgo & mo) == mo;
 #pragma unroll
-  for (int r = 0; r < kRegAmps; r++) {
+  for (int r = 0; r < (1 << G); r++) {
     const bool flip = th && ((uint32_t)r & rm) == rm;
     a[r].x = flip ? -a[r].x : a[r].x;
     a[r].y = flip ? -a[r].y : a[r].y;
   }
 }
 
-template <typename T>
-__device__ __forceinline__ void op_phase(C2<T> (&a)[kRegAmps], const DevOp* __restrict__ op,
+template <int K, int N>
+__device__ __forceinline__ void acc_add(double (&acc)[N], uint32_t cls, double add) {
+  if constexpr (K < N) {
+    if (cls == (uint32_t)K) acc[K] += add;
+    else acc_add<K + 1, N>(acc, cls, add);
+  }
+}
+
+template <typename T, int G>
+__device__ __forceinline__ void op_phase(C2<T> (&a)[1 << G], const DevOp* __restrict__ op,
                                          const PhaseTerm* __restrict__ terms, uint64_t tgo) {
-  // Terms touching at most two register bits fold into 11 accumulators
-  // (constant, four linear, six pair coefficients); the rare terms on three
-  // or more register bits (class 11) are stored after them and are
-  // evaluated per amplitude.
-  double c0 = 0.0;
-  double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
-  double p01 = 0.0, p02 = 0.0, p03 = 0.0, p12 = 0.0, p13 = 0.0, p23 = 0.0;
+  // Terms on at most two register bits fold into 1 + G + G(G-1)/2
+  // accumulators (PhaseSlots); the rare terms on three or more register bits
+  // are stored after them (op->b of them) and are evaluated per amplitude.
+  using L = PhaseSlots<G>;
+  double acc[L::kCount];
+#pragma unroll
+  for (int k = 0; k < L::kCount; k++) acc[k] = 0.0;
   const uint32_t end = op->term_off + op->n_terms;
   const uint32_t gen_end = end + op->b;
   for (uint32_t t = op->term_off; t < end; t++) {
     const PhaseTerm* pt = terms + t;
     const uint64_t mo = pt->mask_out;
     const double add = ((tgo & mo) == mo) ? pt->angle : 0.0;
-    switch (pt->cls) {
-      case 0: c0 += add; break;
-      case 1: l0 += add; break;
-      case 2: l1 += add; break;
-      case 3: l2 += add; break;
-      case 4: l3 += add; break;
-      case 5: p01 += add; break;
-      case 6: p02 += add; break;
-      case 7: p03 += add; break;
-      case 8: p12 += add; break;
-      case 9: p13 += add; break;
-      default: p23 += add; break;
-    }
+    acc_add<0, L::kCount>(acc, pt->cls, add);
   }
 #pragma unroll
-  for (int r = 0; r < kRegAmps; r++) {
-    double phi = c0;
-    if (r & 1) phi += l0;
-    if (r & 2) phi += l1;
-    if (r & 4) phi += l2;
-    if (r & 8) phi += l3;
-    if ((r & 3) == 3) phi += p01;
-    if ((r & 5) == 5) phi += p02;
-    if ((r & 9) == 9) phi += p03;
-    if ((r & 6) == 6) phi += p12;
-    if ((r & 10) == 10) phi += p13;
-    if ((r & 12) == 12) phi += p23;
+  for (int r = 0; r < (1 << G); r++) {
+    double phi = acc[0];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      if (r & (1 << g)) phi += acc[L::linear(g)];
+#pragma unroll
+      for (int h = g + 1; h < G; h++)
+        if ((r & (1 << g)) && (r & (1 << h))) phi += acc[L::pair(g, h)];
+    }
     for (uint32_t t = end; t < gen_end; t++) {
       const PhaseTerm* pt = terms + t;
       const uint64_t mo = pt->mask_out;
@@ -190,52 +226,72 @@ __device__ __forceinline__ void op_phase(C2<T> (&a)[kRegAmps], const DevOp* __re
   }
 }
 
-template <typename T, int FEAT>
-__device__ __forceinline__ void run_ops(C2<T> (&a)[kRegAmps], const DevOp* __restrict__ ops,
-                                        const PhaseTerm* __restrict__ terms, uint32_t begin, uint32_t end,
-                                        uint64_t tgo, const uint64_t (&roff)[kRegAmps]) {
-  for (uint32_t o = begin; o < end; o++) {
-    const DevOp* op = ops + o;
-    const uint32_t type = op->type;
-    if constexpr ((FEAT & kFeat2Q) == 0) {
-      if (type == OP_2Q) continue;
-    }
-    if constexpr ((FEAT & kFeatPhase) == 0) {
-      if (type == OP_PHASE) continue;
-    }
-    switch (type) {
-      case OP_1Q:
-        switch (op->a) {
-          case 0: op_1q<T, 0>(a, op, tgo, roff); break;
-          case 1: op_1q<T, 1>(a, op, tgo, roff); break;
-          case 2: op_1q<T, 2>(a, op, tgo, roff); break;
-          default: op_1q<T, 3>(a, op, tgo, roff); break;
-        }
-        break;
-      case OP_2Q:
-        if constexpr ((FEAT & kFeat2Q) != 0) switch (op->a * 4 + op->b) {
-          case 1 * 4 + 0: op_2q<T, 1, 0>(a, op, tgo, roff); break;
-          case 2 * 4 + 0: op_2q<T, 2, 0>(a, op, tgo, roff); break;
-          case 3 * 4 + 0: op_2q<T, 3, 0>(a, op, tgo, roff); break;
-          case 2 * 4 + 1: op_2q<T, 2, 1>(a, op, tgo, roff); break;
-          case 3 * 4 + 1: op_2q<T, 3, 1>(a, op, tgo, roff); break;
-          default: op_2q<T, 3, 2>(a, op, tgo, roff); break;
-        }
-        break;
-      case OP_DIAG: op_diag<T>(a, op, tgo, roff); break;
-      case OP_SIGN: op_sign<T>(a, op, tgo); break;
-      default:
-        if constexpr ((FEAT & kFeatPhase) != 0) op_phase<T>(a, op, terms, tgo);
-        break;
+template <typename T, int G, int B>
+__device__ __forceinline__ void dispatch_1q(C2<T> (&a)[1 << G], const DevOp* op, uint64_t tgo,
+                                            const uint64_t (&rg)[G], uint32_t b) {
+  if constexpr (B < G) {
+    if (b == (uint32_t)B) op_1q<T, G, B>(a, op, tgo, rg);
+    else dispatch_1q<T, G, B + 1>(a, op, tgo, rg, b);
+  }
+}
+
+template <typename T, int G, int A, int B>
+__device__ __forceinline__ void dispatch_2q(C2<T> (&a)[1 << G], const DevOp* op, uint64_t tgo,
+                                            const uint64_t (&rg)[G], uint32_t ab) {
+  if constexpr (A < G) {
+    if constexpr (B < A) {
+      if (ab == (uint32_t)(A * 8 + B)) op_2q<T, G, A, B>(a, op, tgo, rg);
+      else dispatch_2q<T, G, A, B + 1>(a, op, tgo, rg, ab);
+    } else {
+      dispatch_2q<T, G, A + 1, 0>(a, op, tgo, rg, ab);
     }
   }
 }
 
-template <typename T, int SW, int BT, int FEAT>
-__global__ void __launch_bounds__(BT, BT == 256 ? 2 : 1) k_fused_pass(C2<T>* __restrict__ state, const PassDev* __restrict__ pd,
-                                                     const StageDev* __restrict__ stages,
-                                                     const DevOp* __restrict__ ops,
-                                                     const PhaseTerm* __restrict__ terms) {
+template <typename T, int G, int FEAT>
+__device__ __forceinline__ void run_ops(C2<T> (&a)[1 << G], const DevOp* __restrict__ ops,
+                                        const PhaseTerm* __restrict__ terms, uint32_t begin, uint32_t end,
+                                        uint64_t tgo, const uint64_t (&rg)[G]) {
+  for (uint32_t o = begin; o < end; o++) {
+    const DevOp* op = ops + o;
+    const uint32_t type = op->type;
+    if (type == OP_1Q) {
+      dispatch_1q<T, G, 0>(a, op, tgo, rg, op->a);
+    } else if (type == OP_SIGN) {
+      op_sign<T, G>(a, op, tgo);
+    } else if (type == OP_DIAG) {
+      op_diag<T, G>(a, op, tgo, rg);
+    } else if (type == OP_2Q) {
+      if constexpr ((FEAT & kFeat2Q) != 0) dispatch_2q<T, G, 1, 0>(a, op, tgo, rg, op->a * 8 + op->b);
+    } else {
+      if constexpr ((FEAT & kFeatPhase) != 0) op_phase<T, G>(a, op, terms, tgo);
+    }
+  }
+}
+
+typedef double v2f64 __attribute__((ext_vector_type(2)));
+typedef float v2f32 __attribute__((ext_vector_type(2)));
+
+template <bool NT, typename V>
+__device__ __forceinline__ V gload(const V* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
+template <bool NT, typename V>
+__device__ __forceinline__ void gstore(V* p, V v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+template <typename T, int G, int BT, int FEAT, bool NT>
+__global__ void __launch_bounds__(BT, 2) k_fused_pass(C2<T>* __restrict__ state, const PassDev* __restrict__ pd,
+                                                      const StageDev* __restrict__ stages,
+                                                      const DevOp* __restrict__ ops,
+                                                      const PhaseTerm* __restrict__ terms) {
+  constexpr int SW = sizeof(T) == 8 ? 3 : 4;
+  constexpr int N = 1 << G;
+  using V = typename std::conditional<sizeof(T) == 8, v2f64, v2f32>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   C2<T>* lds = reinterpret_cast<C2<T>*>(smem);
   const uint32_t m = pd->m;
@@ -243,7 +299,7 @@ __global__ void __launch_bounds__(BT, BT == 256 ? 2 : 1) k_fused_pass(C2<T>* __r
   for (uint32_t i = 0; i < m; i++) base = insert_zero(base, pd->local_pos[i]);
   const uint32_t tid = threadIdx.x;
   const uint32_t ns = pd->n_stages;
-  C2<T> a[kRegAmps];
+  C2<T> a[N];
   for (uint32_t s = 0; s < ns; s++) {
     const StageDev* sd = stages + pd->stage_begin + s;
     uint64_t tgo = base;
@@ -255,30 +311,37 @@ __global__ void __launch_bounds__(BT, BT == 256 ? 2 : 1) k_fused_pass(C2<T>* __r
         tlo |= sd->tl[j];
       }
     }
-    const uint64_t rg0 = sd->rg[0], rg1 = sd->rg[1], rg2 = sd->rg[2], rg3 = sd->rg[3];
-    const uint32_t rl0 = sd->rl[0], rl1 = sd->rl[1], rl2 = sd->rl[2], rl3 = sd->rl[3];
-    uint64_t roff[kRegAmps];
-    uint32_t loff[kRegAmps];
+    uint64_t rg[G];
+    uint32_t rl[G];
 #pragma unroll
-    for (int r = 0; r < kRegAmps; r++) {
-      roff[r] = ((r & 1) ? rg0 : 0) | ((r & 2) ? rg1 : 0) | ((r & 4) ? rg2 : 0) | ((r & 8) ? rg3 : 0);
-      loff[r] = ((r & 1) ? rl0 : 0) | ((r & 2) ? rl1 : 0) | ((r & 4) ? rl2 : 0) | ((r & 8) ? rl3 : 0);
+    for (int g = 0; g < G; g++) {
+      rg[g] = sd->rg[g];
+      rl[g] = sd->rl[g];
     }
     if (s == 0) {
 #pragma unroll
-      for (int r = 0; r < kRegAmps; r++) a[r] = state[tgo | roff[r]];
+      for (int r = 0; r < N; r++) {
+        const V v = gload<NT>(reinterpret_cast<const V*>(state + (tgo | roff<G>(rg, r))));
+        a[r].x = v.x;
+        a[r].y = v.y;
+      }
     } else {
 #pragma unroll
-      for (int r = 0; r < kRegAmps; r++) a[r] = lds[swz<SW>(tlo | loff[r])];
+      for (int r = 0; r < N; r++) a[r] = lds[swz<SW>(tlo | loff<G>(rl, r))];
     }
-    run_ops<T, FEAT>(a, ops, terms, sd->op_begin, sd->op_end, tgo, roff);
+    run_ops<T, G, FEAT>(a, ops, terms, sd->op_begin, sd->op_end, tgo, rg);
     if (s + 1 == ns) {
 #pragma unroll
-      for (int r = 0; r < kRegAmps; r++) state[tgo | roff[r]] = a[r];
+      for (int r = 0; r < N; r++) {
+        V v;
+        v.x = a[r].x;
+        v.y = a[r].y;
+        gstore<NT>(reinterpret_cast<V*>(state + (tgo | roff<G>(rg, r))), v);
+      }
     } else {
       if (s > 0) __syncthreads();
 #pragma unroll
-      for (int r = 0; r < kRegAmps; r++) lds[swz<SW>(tlo | loff[r])] = a[r];
+      for (int r = 0; r < N; r++) lds[swz<SW>(tlo | loff<G>(rl, r))] = a[r];
       __syncthreads();
     }
   }
@@ -286,46 +349,64 @@ __global__ void __launch_bounds__(BT, BT == 256 ? 2 : 1) k_fused_pass(C2<T>* __r
 
 }  // namespace
 
-template <typename T, int BT, int FEAT>
+template <typename T, int G, int BT, int FEAT, bool NT>
 static void launch_bt(hipStream_t stream, void* state, uint32_t n, uint32_t m, uint32_t n_stages, const PassDev* pd,
                       const StageDev* stages, const DevOp* ops, const PhaseTerm* terms) {
-  constexpr int SW = sizeof(T) == 8 ? 3 : 4;
   const uint64_t blocks = 1ull << (n - m);
-  const uint32_t threads = 1u << (m - kRegBits);
+  const uint32_t threads = 1u << (m - G);
   const size_t lds = n_stages > 1 ? (sizeof(C2<T>) << m) : 0;
   static bool configured = false;
   if (!configured) {
-    ALEPH_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_pass<T, SW, BT, FEAT>),
+    ALEPH_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_pass<T, G, BT, FEAT, NT>),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     configured = true;
   }
-  hipLaunchKernelGGL((k_fused_pass<T, SW, BT, FEAT>), dim3((uint32_t)blocks), dim3(threads), lds, stream,
+  hipLaunchKernelGGL((k_fused_pass<T, G, BT, FEAT, NT>), dim3((uint32_t)blocks), dim3(threads), lds, stream,
                      reinterpret_cast<C2<T>*>(state), pd, stages, ops, terms);
   ALEPH_HIP_CHECK(hipGetLastError());
 }
 
-template <typename T, int FEAT>
-static void launch_feat(hipStream_t stream, void* state, uint32_t n, uint32_t m, uint32_t n_stages, const PassDev* pd,
-                        const StageDev* stages, const DevOp* ops, const PhaseTerm* terms) {
-  const uint32_t threads = 1u << (m - kRegBits);
-  if (threads <= 256) launch_bt<T, 256, FEAT>(stream, state, n, m, n_stages, pd, stages, ops, terms);
-  else launch_bt<T, 512, FEAT>(stream, state, n, m, n_stages, pd, stages, ops, terms);
+template <typename T, int G, int FEAT, bool NT>
+static void launch_threads(hipStream_t stream, void* state, uint32_t n, uint32_t m, uint32_t n_stages,
+                           const PassDev* pd, const StageDev* stages, const DevOp* ops, const PhaseTerm* terms) {
+  const uint32_t threads = 1u << (m - G);
+  if (threads <= 64) launch_bt<T, G, 64, FEAT, NT>(stream, state, n, m, n_stages, pd, stages, ops, terms);
+  else if (threads <= 256) launch_bt<T, G, 256, FEAT, NT>(stream, state, n, m, n_stages, pd, stages, ops, terms);
+  else launch_bt<T, G, 512, FEAT, NT>(stream, state, n, m, n_stages, pd, stages, ops, terms);
+}
+
+template <typename T, int G, bool NT>
+static void launch_feat(hipStream_t stream, void* state, uint32_t n, uint32_t m, uint32_t n_stages, uint32_t feat,
+                        const PassDev* pd, const StageDev* stages, const DevOp* ops, const PhaseTerm* terms) {
+  switch (feat & (kFeat2Q | kFeatPhase)) {
+    case 0: launch_threads<T, G, 0, NT>(stream, state, n, m, n_stages, pd, stages, ops, terms); break;
+    case kFeat2Q: launch_threads<T, G, kFeat2Q, NT>(stream, state, n, m, n_stages, pd, stages, ops, terms); break;
+    case kFeatPhase:
+      launch_threads<T, G, kFeatPhase, NT>(stream, state, n, m, n_stages, pd, stages, ops, terms);
+      break;
+    default:
+      launch_threads<T, G, kFeat2Q | kFeatPhase, NT>(stream, state, n, m, n_stages, pd, stages, ops, terms);
+      break;
+  }
 }
 
 template <typename T>
 void launch_fused_pass(hipStream_t stream, void* state, uint32_t n, uint32_t m, uint32_t n_stages, uint32_t feat,
-                       const PassDev* pd, const StageDev* stages, const DevOp* ops, const PhaseTerm* terms) {
-  switch (feat & (kFeat2Q | kFeatPhase)) {
-    case 0: launch_feat<T, 0>(stream, state, n, m, n_stages, pd, stages, ops, terms); break;
-    case kFeat2Q: launch_feat<T, kFeat2Q>(stream, state, n, m, n_stages, pd, stages, ops, terms); break;
-    case kFeatPhase: launch_feat<T, kFeatPhase>(stream, state, n, m, n_stages, pd, stages, ops, terms); break;
-    default: launch_feat<T, kFeat2Q | kFeatPhase>(stream, state, n, m, n_stages, pd, stages, ops, terms); break;
+                       uint32_t reg_bits, const PassDev* pd, const StageDev* stages, const DevOp* ops,
+                       const PhaseTerm* terms) {
+  const bool nt = (feat & kFeatNontemporal) != 0;
+  if (reg_bits == 5) {
+    if (nt) launch_feat<T, 5, true>(stream, state, n, m, n_stages, feat, pd, stages, ops, terms);
+    else launch_feat<T, 5, false>(stream, state, n, m, n_stages, feat, pd, stages, ops, terms);
+  } else {
+    if (nt) launch_feat<T, 4, true>(stream, state, n, m, n_stages, feat, pd, stages, ops, terms);
+    else launch_feat<T, 4, false>(stream, state, n, m, n_stages, feat, pd, stages, ops, terms);
   }
 }
 
-template void launch_fused_pass<double>(hipStream_t, void*, uint32_t, uint32_t, uint32_t, uint32_t, const PassDev*,
-                                        const StageDev*, const DevOp*, const PhaseTerm*);
-template void launch_fused_pass<float>(hipStream_t, void*, uint32_t, uint32_t, uint32_t, uint32_t, const PassDev*,
-                                       const StageDev*, const DevOp*, const PhaseTerm*);
+template void launch_fused_pass<double>(hipStream_t, void*, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
+                                        const PassDev*, const StageDev*, const DevOp*, const PhaseTerm*);
+template void launch_fused_pass<float>(hipStream_t, void*, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
+                                       const PassDev*, const StageDev*, const DevOp*, const PhaseTerm*);
 
 }  // namespace aleph

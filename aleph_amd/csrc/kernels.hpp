@@ -40,7 +40,8 @@ void launch_narrow_f64(hipStream_t s, const void* src_f64, void* dst_f32, uint64
 // ---- fused pass (k_fused.hip) ----
 template <typename T>
 void launch_fused_pass(hipStream_t s, void* state, uint32_t n, uint32_t m, uint32_t n_stages, uint32_t features,
-                       const PassDev* pd, const StageDev* stages, const DevOp* ops, const PhaseTerm* terms);
+                       uint32_t reg_bits, const PassDev* pd, const StageDev* stages, const DevOp* ops,
+                       const PhaseTerm* terms);
 
 // ---- readout (k_readout.hip) ----
 // Result layout in `out` (device, 3 doubles): [sum of |a|^2 over bit q clear,

@@ -15,17 +15,17 @@
 
 namespace aleph {
 
-constexpr int kRegBits = 4;                 // G: amplitudes per thread = 16
-constexpr int kRegAmps = 1 << kRegBits;
-constexpr int kMaxLocalBits = 13;           // chunk <= 2^13 amplitudes (512 threads)
-constexpr int kMaxThreadBits = kMaxLocalBits - kRegBits;
+constexpr int kRegBits = 4;                 // default G: amplitudes per thread = 2^G
+constexpr int kMaxRegBits = 5;              // G is 4 or 5
+constexpr int kMaxLocalBits = 13;           // chunk <= 2^13 amplitudes
+constexpr int kMaxThreadBits = 10;          // slots for thread-id bits (m - G <= 9)
 
 enum OpType : uint32_t {
   OP_1Q = 1,     // 2x2 on register bit a; optional control mask (global index)
   OP_2Q = 2,     // 4x4 on register bits (a = matrix MSB, b = LSB), a > b
   OP_DIAG = 3,   // diagonal on up to two qubits given as global masks ma (MSB), mb
   OP_SIGN = 4,   // negate where (idx & mask) == mask; thread part in `cmask`, register part in `a`
-  OP_PHASE = 5,  // multiply by exp(i*phi): terms[term_off, +n_terms) of classes 0..10, then `b` class-11 terms
+  OP_PHASE = 5,  // multiply by exp(i*phi): terms[term_off, +n_terms) of PhaseSlots classes, then `b` general terms
 };
 
 struct alignas(16) DevOp {
@@ -39,10 +39,18 @@ struct alignas(16) DevOp {
   double m[32];       // OP_1Q: 2x2, OP_2Q: 4x4, OP_DIAG: 4 entries (interleaved complex)
 };
 
+// Accumulator slots of OP_PHASE for G register bits: 0 constant, then one
+// per register bit (linear), then one per register pair (g < h).
+template <int G>
+struct PhaseSlots {
+  static constexpr int kCount = 1 + G + G * (G - 1) / 2;
+  static constexpr int linear(int g) { return 1 + g; }
+  static constexpr int pair(int g, int h) { return 1 + G + g * G - g * (g + 1) / 2 + (h - g - 1); }
+};
+
 // Phase term: contributes `angle` when (thread_index & mask_out) == mask_out
-// and (reg & rmask) == rmask.  `cls` selects the accumulator:
-// 0 constant, 1..4 linear in register bit cls-1, 5..10 register pair,
-// 11 general (three or four register bits).
+// and (reg & rmask) == rmask.  `cls` is its PhaseSlots accumulator; terms
+// on three or more register bits are evaluated per amplitude instead.
 struct alignas(16) PhaseTerm {
   uint64_t mask_out;
   double angle;
@@ -58,14 +66,16 @@ struct alignas(16) StageDev {
   uint64_t tg[kMaxThreadBits];   // physical mask of thread-id bit j
   uint32_t tl[kMaxThreadBits];   // chunk-local mask of thread-id bit j
   uint32_t pad2[2];
-  uint64_t rg[kRegBits];         // physical mask of register bit g
-  uint32_t rl[kRegBits];         // chunk-local mask of register bit g
+  uint64_t rg[kMaxRegBits];      // physical mask of register bit g
+  uint32_t rl[kMaxRegBits];      // chunk-local mask of register bit g
+  uint32_t pad3;
 };
 
 // Op families a pass uses; the launcher picks the kernel variant compiled
 // for exactly those, keeping register pressure of unused paths out.
 constexpr uint32_t kFeat2Q = 1;
 constexpr uint32_t kFeatPhase = 2;
+constexpr uint32_t kFeatNontemporal = 4;   // stream the chunk with non-temporal loads/stores
 
 struct alignas(16) PassDev {
   uint32_t m;                    // chunk qubits
@@ -73,7 +83,8 @@ struct alignas(16) PassDev {
   uint32_t stage_begin;
   uint32_t features;             // kFeat* bits
   uint32_t local_pos[14];        // ascending physical positions of chunk qubits
-  uint32_t pad2[2];
+  uint32_t reg_bits;             // G of this pass
+  uint32_t pad2;
 };
 
 }  // namespace aleph

@@ -17,12 +17,12 @@ OP_NAMES = {1: "1q", 2: "2q", 3: "diag", 4: "sign", 5: "phase"}
 
 class PassDev(C.Structure):
     _fields_ = [("m", u32), ("n_stages", u32), ("stage_begin", u32), ("features", u32), ("local_pos", u32 * 14),
-                ("pad2", u32 * 2)]
+                ("reg_bits", u32), ("pad2", u32)]
 
 
 class StageDev(C.Structure):
     _fields_ = [("op_begin", u32), ("op_end", u32), ("n_thread_bits", u32), ("pad", u32), ("tg", u64 * 10),
-                ("tl", u32 * 10), ("pad2", u32 * 2), ("rg", u64 * 4), ("rl", u32 * 4)]
+                ("tl", u32 * 10), ("pad2", u32 * 2), ("rg", u64 * 5), ("rl", u32 * 5), ("pad3", u32)]
 
 
 class DevOp(C.Structure):
@@ -45,7 +45,8 @@ class PlanView(C.Structure):
                 ("passes", C.c_void_p), ("n_passes", u32), ("pass_bytes", u32),
                 ("stages", C.c_void_p), ("n_stages", u32), ("stage_bytes", u32),
                 ("ops", C.c_void_p), ("n_ops", u32), ("op_bytes", u32),
-                ("terms", C.c_void_p), ("n_terms", u32), ("term_bytes", u32)]
+                ("terms", C.c_void_p), ("n_terms", u32), ("term_bytes", u32),
+                ("payload", C.c_void_p), ("payload_bytes", u64)]
 
 
 def _array(ptr, count, cls, nbytes):
@@ -61,13 +62,15 @@ class HostPlan:
     """A plan built on the host; arrays are copied out on construction."""
 
     def __init__(self, circuit, precision: int = 64, planner: int = L.PLANNER_RULE, local_qubits: int = 0,
-                 max_ops: int = 0, coalesce_bits: int = 0, fuse_1q: bool = True):
+                 max_ops: int = 0, coalesce_bits: int = 0, fuse_1q: bool = True, reg_bits: int = 0, nontemporal: bool = False,
+                fuse_2q: bool = True):
         lib = L.load()
         words, reals = circuit.encode()
         nw, nr = int(words.size), int(reals.size)
         words = np.ascontiguousarray(words if nw else np.zeros(1, np.uint32))
         reals = np.ascontiguousarray(reals if nr else np.zeros(1))
-        opts = L.PlanOpts(planner, local_qubits, max_ops, coalesce_bits, int(fuse_1q))
+        opts = L.PlanOpts(planner, local_qubits, max_ops, coalesce_bits, int(fuse_1q), reg_bits,
+                          (L.PLAN_NONTEMPORAL if nontemporal else 0) | (0 if fuse_2q else L.PLAN_NO_FUSE_2Q))
         h = C.c_void_p()
         err = L.Error()
         code = lib.aleph_hip_plan_build_host(circuit.num_qubits, precision, words.ctypes.data_as(L.PU32), nw,
@@ -85,6 +88,7 @@ class HostPlan:
             self.ops = [DevOp.from_buffer_copy(s) for s in _array(v.ops, v.n_ops, DevOp, v.op_bytes)]
             self.terms = [PhaseTerm.from_buffer_copy(s) for s in _array(v.terms, v.n_terms, PhaseTerm,
                                                                          v.term_bytes)]
+            self.payload = C.string_at(v.payload, v.payload_bytes) if v.payload_bytes else b""
             st = L.PlanStats()
             lib.aleph_hip_plan_stats_get(h, C.byref(st))
             self.stats = {f: getattr(st, f) for f, _ in L.PlanStats._fields_ if f != "reserved"}

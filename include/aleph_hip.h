@@ -177,8 +177,14 @@ typedef struct aleph_hip_plan_opts {
   uint32_t max_ops_per_pass;/* compute budget per pass, in 1q-gate equivalents (0 = default) */
   uint32_t coalesce_bits;   /* low qubits kept in every chunk for coalescing (0 = default) */
   uint32_t fuse_1q;         /* merge adjacent 1q gates on one qubit before planning */
-  uint32_t reserved[3];
+  uint32_t reg_bits;        /* amplitudes per thread = 2^reg_bits, 4 or 5 (0 = default 4) */
+  uint32_t flags;           /* ALEPH_HIP_PLAN_* bits */
+  uint32_t reserved;
 } aleph_hip_plan_opts;
+enum {
+  ALEPH_HIP_PLAN_NONTEMPORAL = 1, /* stream chunks with non-temporal loads/stores */
+  ALEPH_HIP_PLAN_NO_FUSE_2Q = 2   /* keep 1q neighbours out of 2q gates (no 4x4 blocks) */
+};
 
 typedef struct aleph_hip_plan_stats {
   double plan_ms;           /* host planning wall time */
@@ -214,6 +220,7 @@ typedef struct aleph_hip_plan_view {
   const void* stages; uint32_t n_stages; uint32_t stage_bytes;
   const void* ops; uint32_t n_ops; uint32_t op_bytes;
   const void* terms; uint32_t n_terms; uint32_t term_bytes;
+  const void* payload; uint64_t payload_bytes;   /* standalone-step matrices / CSR */
 } aleph_hip_plan_view;
 aleph_hip_status aleph_hip_plan_build_host(uint32_t num_qubits, uint32_t precision, const uint32_t* words,
                                            size_t n_words, const double* reals, size_t n_reals,

@@ -1,0 +1,93 @@
+"""Host-side planner checks (no device): plans built on the host are
+emulated op by op (tests/plan_emulator.py) and compared with the CPU oracle
+at 1e-12; structural properties of the plans are checked directly."""
+
+import math
+
+import numpy as np
+import pytest
+
+from aleph_amd import workloads as W
+from aleph_amd.circuit import Circuit, DiagonalPhase, Gate, PhaseTerm
+from aleph_amd.plan_tools import HostPlan
+from tests.helpers import max_abs
+from tests.plan_emulator import emulate
+from tests.test_oracle import random_circuit
+
+
+def oracle_state(orc, c, psi):
+    o = orc.OracleSv()
+    st = o.allocate(c.num_qubits)
+    st.load(psi)
+    o.run(c, st)
+    return st.amplitudes()
+
+
+OPTION_GRID = [
+    {},
+    {"local_qubits": 4}, {"local_qubits": 6, "coalesce_bits": 1}, {"local_qubits": 9, "max_ops": 2},
+    {"fuse_1q": False, "fuse_2q": False}, {"reg_bits": 5, "local_qubits": 8}, {"max_ops": 500},
+    {"planner": 0},
+]
+
+
+@pytest.mark.parametrize("seed", range(10))
+@pytest.mark.parametrize("opt", range(len(OPTION_GRID)))
+def test_emulated_plan_matches_oracle(oracle_lib, seed, opt):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(5, 10))
+    c = random_circuit(n, 60, rng)
+    psi = W.gaussian_state(n, seed)
+    kw = OPTION_GRID[opt]
+    if kw.get("local_qubits", 0) > n:
+        kw = {**kw, "local_qubits": n}
+    plan = HostPlan(c, **kw)
+    assert max_abs(emulate(plan, psi), oracle_state(oracle_lib, c, psi)) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["brickwall", "qft", "hea", "qaoa", "phase"])
+def test_emulated_workloads(oracle_lib, name):
+    n = 10
+    if name == "brickwall":
+        c = W.brickwall(n, 12, seed=3)
+    elif name == "qft":
+        c = W.qft(n)
+    elif name == "hea":
+        c = W.hea(n, 4, seed=2)
+    elif name == "qaoa":
+        c = W.qaoa_maxcut(n, W.random_3_regular(n, 1), [0.4, 1.3], [0.9, 0.2])
+    else:
+        rng = np.random.default_rng(5)
+        c = Circuit(n).h(0).h(5)
+        c.add(DiagonalPhase(n, [PhaseTerm(tuple(1 << int(q) for q in rng.choice(n, 3, replace=False)), 0.7),
+                                PhaseTerm((0b101,), -0.4), PhaseTerm((), 0.3)]))
+        c.cx(5, 2).h(2)
+    psi = W.gaussian_state(n, 9)
+    for kw in ({}, {"local_qubits": 6}, {"reg_bits": 5, "local_qubits": 9}):
+        plan = HostPlan(c, **kw)
+        assert max_abs(emulate(plan, psi), oracle_state(oracle_lib, c, psi)) < 1e-12, kw
+
+
+def test_brickwall_pair_fusion_structure():
+    """U3xU3·CZ·U3xU3 blocks on even pairs, odd-layer CZs left as sign flips."""
+    n, depth = 30, 20
+    plan = HostPlan(W.brickwall(n, depth, seed=2024))
+    kinds = [op.type for op in plan.ops]
+    assert kinds.count(2) == (n // 2) * (depth // 2)            # 4x4 blocks
+    assert kinds.count(4) == (n // 2 - 1) * (depth // 2)        # bare odd-layer CZs
+    assert kinds.count(1) == 0
+    unfused = HostPlan(W.brickwall(n, depth, seed=2024), fuse_2q=False)
+    assert plan.stats["n_passes"] < unfused.stats["n_passes"]
+
+
+def test_plan_stages_respect_register_width():
+    plan = HostPlan(W.brickwall(20, 8, seed=1), local_qubits=11)
+    for p in plan.passes:
+        assert p.m == 11
+        assert list(p.local_pos[:p.m]) == sorted(p.local_pos[:p.m])
+        for sd in plan.stages[p.stage_begin:p.stage_begin + p.n_stages]:
+            regs = [int(sd.rg[g]) for g in range(p.reg_bits)]
+            assert len(set(regs)) == p.reg_bits
+            assert sd.n_thread_bits == p.m - p.reg_bits
+            chunk = sum(1 << q for q in p.local_pos[:p.m])
+            assert all(r & chunk for r in regs)
