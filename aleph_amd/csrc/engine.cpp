@@ -669,6 +669,15 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
       if (pend_reg.empty() && pend_gen.empty()) return;
       DevOp ph{};
       ph.type = OP_PHASE;
+      // sort by accumulator class; class boundaries go into the matrix slots
+      std::stable_sort(pend_reg.begin(), pend_reg.end(),
+                       [](const PhaseTerm& x, const PhaseTerm& y) { return x.cls < y.cls; });
+      const uint32_t n_classes = 1 + pc.G + pc.G * (pc.G - 1) / 2;
+      uint32_t* bounds = reinterpret_cast<uint32_t*>(ph.m);
+      for (uint32_t k = 0, t = 0; k <= n_classes; k++) {
+        while (t < pend_reg.size() && pend_reg[t].cls < k) t++;
+        bounds[k] = t;
+      }
       ph.term_off = (uint32_t)p.terms.size();
       ph.n_terms = (uint32_t)pend_reg.size();
       ph.b = (uint32_t)pend_gen.size();

@@ -90,7 +90,7 @@ constexpr int ins0(int x, int p) { return (x & ((1 << p) - 1)) | ((x >> p) << (p
 // copied out, then each output row is formed from one matrix row read just
 // before use.  The empty asm with a memory clobber keeps the compiler from
 // hoisting all sixteen matrix entries into registers at once.
-template <typename T, int G, int A, int B, int Q>
+template <typename T, int G, int A, int B, int Q, bool CTL>
 __device__ __forceinline__ void quartets_2q(C2<T> (&a)[1 << G], const DevOp* __restrict__ op,
                                             const bool (&on)[1 << (G - 2)]) {
   if constexpr (Q < (1 << (G - 2))) {
@@ -123,12 +123,17 @@ __device__ __forceinline__ void quartets_2q(C2<T> (&a)[1 << G], const DevOp* __r
       out[row].x = re;
       out[row].y = im;
     }
-    const bool live = on[Q];
-    a[idx[0]] = live ? out[0] : v0;
-    a[idx[1]] = live ? out[1] : v1;
-    a[idx[2]] = live ? out[2] : v2;
-    a[idx[3]] = live ? out[3] : v3;
-    quartets_2q<T, G, A, B, Q + 1>(a, op, on);
+    if constexpr (CTL) {
+      const bool live = on[Q];
+      a[idx[0]] = live ? out[0] : v0;
+      a[idx[1]] = live ? out[1] : v1;
+      a[idx[2]] = live ? out[2] : v2;
+      a[idx[3]] = live ? out[3] : v3;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) a[idx[k]] = out[k];
+    }
+    quartets_2q<T, G, A, B, Q + 1, CTL>(a, op, on);
   }
 }
 
@@ -136,11 +141,15 @@ template <typename T, int G, int A, int B>  // A = matrix MSB register bit, B = 
 __device__ __forceinline__ void op_2q(C2<T> (&a)[1 << G], const DevOp* __restrict__ op, uint64_t tgo,
                                       const uint64_t (&rg)[G]) {
   const uint64_t cm = op->cmask;
-  const uint64_t tc = tgo & cm;
   bool on[1 << (G - 2)];
+  if (cm == 0) {
+    quartets_2q<T, G, A, B, 0, false>(a, op, on);
+    return;
+  }
+  const uint64_t tc = tgo & cm;
 #pragma unroll
   for (int q = 0; q < (1 << (G - 2)); q++) on[q] = (tc | (roff<G>(rg, ins0(ins0(q, B), A)) & cm)) == cm;
-  quartets_2q<T, G, A, B, 0>(a, op, on);
+  quartets_2q<T, G, A, B, 0, true>(a, op, on);
 }
 
 template <typename T, int G>
@@ -176,32 +185,35 @@ __device__ __forceinline__ void op_sign(C2<T> (&a)[1 << G], const DevOp* __restr
   }
 }
 
-template <int K, int N>
-__device__ __forceinline__ void acc_add(double (&acc)[N], uint32_t cls, double add) {
-  if constexpr (K < N) {
-    if (cls == (uint32_t)K) acc[K] += add;
-    else acc_add<K + 1, N>(acc, cls, add);
+// Sum of the angles of terms [b, e) that fire for this thread.  The loads
+// are independent, so unrolling keeps several scalar loads in flight.
+__device__ __forceinline__ double sum_terms(const PhaseTerm* __restrict__ terms, uint32_t b, uint32_t e,
+                                            uint64_t tgo) {
+  double s = 0.0;
+#pragma unroll 4
+  for (uint32_t t = b; t < e; t++) {
+    const uint64_t mo = terms[t].mask_out;
+    s += ((tgo & mo) == mo) ? terms[t].angle : 0.0;
   }
+  return s;
 }
 
 template <typename T, int G>
 __device__ __forceinline__ void op_phase(C2<T> (&a)[1 << G], const DevOp* __restrict__ op,
                                          const PhaseTerm* __restrict__ terms, uint64_t tgo) {
   // Terms on at most two register bits fold into 1 + G + G(G-1)/2
-  // accumulators (PhaseSlots); the rare terms on three or more register bits
-  // are stored after them (op->b of them) and are evaluated per amplitude.
+  // accumulators (PhaseSlots).  The host sorts them by accumulator and
+  // stores the class boundaries (relative to term_off) in the op's matrix
+  // slots; the rare terms on three or more register bits follow (op->b of
+  // them) and are evaluated per amplitude.
   using L = PhaseSlots<G>;
+  const uint32_t* bounds = reinterpret_cast<const uint32_t*>(op->m);
   double acc[L::kCount];
 #pragma unroll
-  for (int k = 0; k < L::kCount; k++) acc[k] = 0.0;
+  for (int k = 0; k < L::kCount; k++)
+    acc[k] = sum_terms(terms, op->term_off + bounds[k], op->term_off + bounds[k + 1], tgo);
   const uint32_t end = op->term_off + op->n_terms;
   const uint32_t gen_end = end + op->b;
-  for (uint32_t t = op->term_off; t < end; t++) {
-    const PhaseTerm* pt = terms + t;
-    const uint64_t mo = pt->mask_out;
-    const double add = ((tgo & mo) == mo) ? pt->angle : 0.0;
-    acc_add<0, L::kCount>(acc, pt->cls, add);
-  }
 #pragma unroll
   for (int r = 0; r < (1 << G); r++) {
     double phi = acc[0];
