@@ -41,6 +41,7 @@ F64, F32 = 64, 32
 PLANNER_UNFUSED, PLANNER_RULE, PLANNER_LEARNED = 0, 1, 2
 PLAN_NONTEMPORAL = 1
 PLAN_NO_FUSE_2Q = 2
+PLAN_NO_FUSE_DIAG = 4
 INSTR_DIAGONAL_PHASE, INSTR_BARRIER = 64, 65
 
 

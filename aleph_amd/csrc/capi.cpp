@@ -575,6 +575,7 @@ std::unique_ptr<aleph_hip_plan> make_plan(uint32_t n, uint32_t precision, const 
       po.reg_bits = opts->reg_bits;
       po.nontemporal = (opts->flags & ALEPH_HIP_PLAN_NONTEMPORAL) != 0;
       po.fuse_2q = (opts->flags & ALEPH_HIP_PLAN_NO_FUSE_2Q) == 0;
+      po.fuse_diag = (opts->flags & ALEPH_HIP_PLAN_NO_FUSE_DIAG) == 0;
     }
     need(po.reg_bits == 0 || po.reg_bits == 4 || po.reg_bits == 5, "reg_bits must be 4 or 5");
     need(po.planner <= ALEPH_HIP_PLANNER_LEARNED, "unknown planner");

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 
 #include "kernels.hpp"
 
@@ -148,6 +149,44 @@ void diag_to_terms(const DenseGate& d, uint64_t cm, std::vector<std::pair<uint64
   }
 }
 
+// Expand one parity term (fires when popcount(c & x) is odd for every c in
+// conds) whose support has at most kMaxParitySupport bits into AND-mask
+// monomials: the indicator is a multilinear polynomial in the support bits,
+// and its coefficients are the Moebius transform of its values on subsets.
+// An empty condition list fires everywhere (a constant phase, mask 0).
+constexpr int kMaxParitySupport = 6;
+
+bool parity_term_to_and(double angle, const std::vector<uint64_t>& conds, std::map<uint64_t, double>& acc) {
+  uint64_t support = 0;
+  for (uint64_t c : conds) support |= c;
+  const int s = popc(support);
+  if (s > kMaxParitySupport) return false;
+  uint32_t pos[kMaxParitySupport];
+  int k = 0;
+  for (uint64_t rest = support; rest; rest &= rest - 1) pos[k++] = (uint32_t)__builtin_ctzll(rest);
+  const uint32_t dim = 1u << s;
+  auto deposit = [&](uint32_t u) {
+    uint64_t x = 0;
+    for (int i = 0; i < s; i++)
+      if ((u >> i) & 1u) x |= 1ull << pos[i];
+    return x;
+  };
+  double a[1 << kMaxParitySupport];
+  for (uint32_t u = 0; u < dim; u++) {
+    const uint64_t x = deposit(u);
+    bool fires = true;
+    for (uint64_t c : conds)
+      if ((popc(c & x) & 1) == 0) fires = false;
+    a[u] = fires ? 1.0 : 0.0;
+  }
+  for (int i = 0; i < s; i++)
+    for (uint32_t u = 0; u < dim; u++)
+      if ((u >> i) & 1u) a[u] -= a[u ^ (1u << i)];
+  for (uint32_t u = 0; u < dim; u++)
+    if (a[u] != 0.0) acc[deposit(u)] += angle * a[u];
+  return true;
+}
+
 LGate lower_gate(const DenseGate& d) {
   LGate g;
   const uint64_t cm = qmask(d.controls);
@@ -274,36 +313,46 @@ std::vector<LGate> lower_program(const std::vector<ProgInstr>& prog, uint32_t n)
       out.push_back(std::move(g));
       continue;
     }
-    // DiagonalPhase: single-bit parity conditions are AND masks.
-    LGate g;
-    bool simple = true;
-    for (const auto& cs : pi.conds)
-      for (uint64_t c : cs)
-        if (popc(c) != 1) simple = false;
-    uint64_t support = 0;
-    for (const auto& cs : pi.conds)
-      for (uint64_t c : cs) support |= c;
-    for (uint32_t q = 0; q < 64 && q < n; q++)
-      if ((support >> q) & 1ull) g.dg.push_back(q);
-    if (simple) {
-      g.kind = LGate::TERMS;
-      for (size_t t = 0; t < pi.angles.size(); t++) {
-        uint64_t m = 0;
-        for (uint64_t c : pi.conds[t]) m |= c;
-        g.terms.emplace_back(m, pi.angles[t]);
+    // DiagonalPhase: terms on few bits become AND-mask terms for the fused
+    // passes; the rest stay a standalone parity-polynomial sweep.  Both
+    // parts are diagonal, so their order does not matter.
+    std::map<uint64_t, double> acc;
+    LGate poly;
+    poly.kind = LGate::POLY;
+    poly.poly_off.push_back(0);
+    uint64_t and_support = 0, poly_support = 0;
+    for (size_t t = 0; t < pi.angles.size(); t++) {
+      if (parity_term_to_and(pi.angles[t], pi.conds[t], acc)) {
+        for (uint64_t c : pi.conds[t]) and_support |= c;
+        continue;
       }
-      g.cost = 0.05 * (double)g.terms.size();
-    } else {
-      g.kind = LGate::POLY;
-      g.poly_off.push_back(0);
-      for (size_t t = 0; t < pi.angles.size(); t++) {
-        for (uint64_t c : pi.conds[t]) g.poly_conds.push_back(c);
-        g.poly_off.push_back((uint32_t)g.poly_conds.size());
-        g.poly_angles.push_back(pi.angles[t]);
+      for (uint64_t c : pi.conds[t]) {
+        poly.poly_conds.push_back(c);
+        poly_support |= c;
       }
-      g.cost = 1e9;
+      poly.poly_off.push_back((uint32_t)poly.poly_conds.size());
+      poly.poly_angles.push_back(pi.angles[t]);
     }
-    out.push_back(std::move(g));
+    auto support_list = [&](uint64_t sup) {
+      std::vector<uint32_t> v;
+      for (uint32_t q = 0; q < 64 && q < n; q++)
+        if ((sup >> q) & 1ull) v.push_back(q);
+      return v;
+    };
+    LGate g;
+    g.kind = LGate::TERMS;
+    for (const auto& kv : acc)
+      if (kv.second != 0.0) g.terms.emplace_back(kv.first, kv.second);
+    if (!g.terms.empty()) {
+      g.dg = support_list(and_support);
+      g.cost = 0.05 * (double)g.terms.size();
+      out.push_back(std::move(g));
+    }
+    if (!poly.poly_angles.empty()) {
+      poly.dg = support_list(poly_support);
+      poly.cost = 1e9;
+      out.push_back(std::move(poly));
+    }
   }
   return out;
 }
@@ -863,6 +912,76 @@ void plan_unfused(Plan& p, const std::vector<ProgInstr>& prog) {
 
 }  // namespace
 
+std::vector<ProgInstr> fuse_diagonal_runs(const std::vector<ProgInstr>& prog, uint32_t n) {
+  // Runs of {diagonal gate, bare CNOT, DiagonalPhase} that contain a CNOT
+  // and whose CNOTs compose to the identity become one DiagonalPhase: each
+  // physical qubit carries the parity mask of input bits it holds, and a
+  // diagonal gate's Moebius terms become parity terms over those masks
+  // (the run semantics of crates/aleph-ir/src/passes/fuse_diagonal.rs:1-80).
+  auto is_cnot = [](const ProgInstr& pi) {
+    return pi.type == ProgInstr::GATE && pi.gate.kind == ALEPH_HIP_GATE_CNOT && pi.gate.controls.empty();
+  };
+  auto member = [&](const ProgInstr& pi) {
+    return pi.type == ProgInstr::DIAG_PHASE || (pi.type == ProgInstr::GATE && (pi.gate.diagonal || is_cnot(pi)));
+  };
+  std::vector<ProgInstr> out;
+  size_t i = 0;
+  while (i < prog.size()) {
+    if (!member(prog[i])) {
+      out.push_back(prog[i++]);
+      continue;
+    }
+    size_t j = i;
+    bool has_cnot = false;
+    while (j < prog.size() && member(prog[j])) has_cnot |= is_cnot(prog[j++]);
+    std::vector<uint64_t> lin(n);
+    for (uint32_t q = 0; q < n; q++) lin[q] = 1ull << q;
+    auto lmask = [&](uint64_t phys) {
+      uint64_t v = 0;
+      for (uint64_t r = phys; r; r &= r - 1) v ^= lin[__builtin_ctzll(r)];
+      return v;
+    };
+    ProgInstr fused;
+    fused.type = ProgInstr::DIAG_PHASE;
+    for (size_t t = i; has_cnot && t < j; t++) {
+      const ProgInstr& pi = prog[t];
+      if (is_cnot(pi)) {
+        lin[pi.gate.qubits[1]] ^= lin[pi.gate.qubits[0]];
+        continue;
+      }
+      if (pi.type == ProgInstr::DIAG_PHASE) {
+        for (size_t u = 0; u < pi.angles.size(); u++) {
+          std::vector<uint64_t> cs;
+          for (uint64_t c : pi.conds[u]) cs.push_back(lmask(c));
+          fused.conds.push_back(std::move(cs));
+          fused.angles.push_back(pi.angles[u]);
+        }
+        continue;
+      }
+      const DenseGate& d = pi.gate;
+      std::vector<std::pair<uint64_t, double>> terms;
+      diag_to_terms(d, qmask(d.controls), terms);
+      // Constant phase of the targets' |0..0> entry: diag_to_terms drops
+      // a zero beta only, so the S = 0 term carries it when nonzero.
+      for (const auto& tm : terms) {
+        std::vector<uint64_t> cs;
+        for (uint64_t r = tm.first; r; r &= r - 1) cs.push_back(lin[__builtin_ctzll(r)]);
+        fused.conds.push_back(std::move(cs));
+        fused.angles.push_back(tm.second);
+      }
+    }
+    bool identity = has_cnot;
+    for (uint32_t q = 0; identity && q < n; q++) identity = lin[q] == (1ull << q);
+    if (identity) {
+      out.push_back(std::move(fused));
+    } else {
+      for (size_t t = i; t < j; t++) out.push_back(prog[t]);
+    }
+    i = j;
+  }
+  return out;
+}
+
 std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog, uint32_t n, uint32_t precision,
                                  const PlanOptions& opts) {
   const auto t0 = std::chrono::steady_clock::now();
@@ -876,7 +995,7 @@ std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog, uint32_t n,
   if (opts.planner == ALEPH_HIP_PLANNER_UNFUSED || n < G) {
     plan_unfused(*p, prog);
   } else {
-    std::vector<LGate> lg = lower_program(prog, n);
+    std::vector<LGate> lg = lower_program(opts.fuse_diag ? fuse_diagonal_runs(prog, n) : prog, n);
     if (opts.fuse_1q) lg = fuse_1q_runs(lg);
     if (opts.fuse_2q) lg = fuse_pairs(lg);
     plan_rule(*p, std::move(lg), n, opts);

@@ -66,6 +66,7 @@ struct PlanOptions {
   uint32_t reg_bits = 0;      // 0: default (4)
   bool nontemporal = false;
   bool fuse_2q = true;        // fuse 1q neighbours into 2q gates (4x4 blocks)
+  bool fuse_diag = true;      // fuse {diagonal, CNOT} runs into one DiagonalPhase
 };
 
 // One device launch of a plan.

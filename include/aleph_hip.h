@@ -183,7 +183,8 @@ typedef struct aleph_hip_plan_opts {
 } aleph_hip_plan_opts;
 enum {
   ALEPH_HIP_PLAN_NONTEMPORAL = 1, /* stream chunks with non-temporal loads/stores */
-  ALEPH_HIP_PLAN_NO_FUSE_2Q = 2   /* keep 1q neighbours out of 2q gates (no 4x4 blocks) */
+  ALEPH_HIP_PLAN_NO_FUSE_2Q = 2,  /* keep 1q neighbours out of 2q gates (no 4x4 blocks) */
+  ALEPH_HIP_PLAN_NO_FUSE_DIAG = 4 /* keep {diagonal, CNOT} runs gate by gate (no DiagonalPhase fusion) */
 };
 
 typedef struct aleph_hip_plan_stats {

@@ -91,3 +91,65 @@ def test_plan_stages_respect_register_width():
             assert sd.n_thread_bits == p.m - p.reg_bits
             chunk = sum(1 << q for q in p.local_pos[:p.m])
             assert all(r & chunk for r in regs)
+
+
+def _random_sandwich(n, rng):
+    """CNOT networks that compose to the identity around diagonal gates and
+    DiagonalPhase terms, between dense layers."""
+    c = Circuit(n)
+    for q in range(n):
+        c.h(q)
+    for _ in range(3):
+        cx = [tuple(int(v) for v in rng.choice(n, 2, replace=False)) for _ in range(int(rng.integers(1, 4)))]
+        for a, b in cx:
+            c.cx(a, b)
+        c.rz(float(rng.uniform(0, 6.3)), int(rng.integers(n)))
+        c.cz(*[int(v) for v in rng.choice(n, 2, replace=False)])
+        c.cp(float(rng.uniform(0, 6.3)), *[int(v) for v in rng.choice(n, 2, replace=False)])
+        c.ccz(*[int(v) for v in rng.choice(n, 3, replace=False)])
+        terms = [PhaseTerm(tuple(int(m) for m in rng.integers(1, 1 << n, size=int(rng.integers(0, 3)))),
+                           float(rng.uniform(-3, 3))) for _ in range(3)]
+        c.add(DiagonalPhase(n, terms))
+        c.t(int(rng.integers(n)))
+        for a, b in reversed(cx):
+            c.cx(a, b)
+        for q in range(n):
+            c.ry(float(rng.uniform(0, 6.3)), q)
+    return c
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_diagonal_run_fusion_matches_oracle(oracle_lib, seed):
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(6, 10))
+    c = _random_sandwich(n, rng)
+    psi = W.gaussian_state(n, seed)
+    want = oracle_state(oracle_lib, c, psi)
+    for kw in ({}, {"local_qubits": 5}, {"fuse_diag": False}):
+        plan = HostPlan(c, **kw)
+        assert max_abs(emulate(plan, psi), want) < 1e-12, kw
+
+
+def test_qaoa_cost_layer_becomes_phase_terms(oracle_lib):
+    """CNOT·Rz·CNOT per edge compose to one DiagonalPhase per layer, which the
+    fused passes apply as AND-mask phase terms: no controlled-X ops remain."""
+    n = 12
+    c = W.qaoa_maxcut(n, W.random_3_regular(n, 4), [0.3, 1.1, 2.0], [0.7, 0.1, 1.9])
+    fused = HostPlan(c)
+    plain = HostPlan(c, fuse_diag=False)
+    assert not any(op.type == 1 and op.cmask for op in fused.ops)
+    assert any(op.type == 1 and op.cmask for op in plain.ops)
+    assert sum(op.type == 5 for op in fused.ops) >= 3
+    assert fused.stats["n_passes"] <= plain.stats["n_passes"]
+    psi = W.gaussian_state(n, 2)
+    want = oracle_state(oracle_lib, c, psi)
+    for plan in (fused, plain):
+        assert max_abs(emulate(plan, psi), want) < 1e-12
+
+
+def test_non_identity_cnot_runs_are_kept():
+    """A CNOT ladder (HEA) does not compose to the identity: the run stays
+    gate by gate and the plan is the same with and without run fusion."""
+    c = W.hea(10, 3, seed=1)
+    a, b = HostPlan(c), HostPlan(c, fuse_diag=False)
+    assert [op.type for op in a.ops] == [op.type for op in b.ops]
