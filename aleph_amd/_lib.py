@@ -65,6 +65,38 @@ class PlanStats(C.Structure):
                 ("n_launches", u32), ("reserved", u32), ("sweep_bytes", f64)]
 
 
+BATCH_PERSISTENT, BATCH_ROUNDS = 0, 1
+BATCH_NONTEMPORAL = 1
+BATCH_UNREACHABLE_DEPS = 2
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("mode", u32), ("local_qubits", u32), ("chunks_per_item", u32), ("groups_per_cu", u32),
+                ("max_ops_per_pass", u32), ("spin_limit", u32), ("flags", u32), ("reserved", u32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("plan_ms", f64), ("sweep_bytes", f64), ("pool_bytes", u64), ("n_instances", u32),
+                ("n_passes", u32), ("n_items", u32), ("n_rounds", u32), ("grid", u32), ("threads", u32),
+                ("lds_bytes", u32), ("mode", u32), ("local_qubits", u32), ("chunks_per_item", u32),
+                ("groups_per_cu", u32), ("reserved", u32)]
+
+
+class BatchTune(C.Structure):
+    _fields_ = [("mode", u32), ("local_qubits", u32), ("chunks_per_item", u32), ("groups_per_cu", u32),
+                ("ms", f64)]
+
+
+class BatchView(C.Structure):
+    _fields_ = [("items", P), ("n_items", u32), ("item_bytes", u32),
+                ("passes", P), ("n_passes", u32), ("pass_bytes", u32),
+                ("stages", P), ("n_stages", u32), ("stage_bytes", u32),
+                ("ops", P), ("n_ops", u32), ("op_bytes", u32),
+                ("terms", P), ("n_terms", u32), ("term_bytes", u32),
+                ("offsets", PU64), ("n_instances", u32), ("n_rounds", u32),
+                ("round_begin", PU32), ("threads", u32), ("lds_bytes", u32), ("total_amps", u64)]
+
+
 # name -> (restype, argtypes); every symbol include/aleph_hip.h declares.
 SIGNATURES = {
     "aleph_hip_device_count": (i32, [PI32]),
@@ -110,6 +142,21 @@ SIGNATURES = {
     "aleph_hip_sample_with_uniforms": (i32, [P, P, PF64, u32, PU64]),
     "aleph_hip_expectation": (i32, [P, P, f64, PU32, PU8, u32, PF64]),
     "aleph_hip_probabilities": (i32, [P, P, PU32, u32, PF64]),
+    "aleph_hip_batch_create": (i32, [P, u32, PU32, u32, PU32, PU64, PF64, PU64, C.POINTER(BatchOpts),
+                                     C.POINTER(P)]),
+    "aleph_hip_batch_create_qaoa": (i32, [P, u32, PU32, PU64, PU32, u32, PF64, PF64, u32, C.POINTER(BatchOpts),
+                                          C.POINTER(P)]),
+    "aleph_hip_batch_set_qaoa_angles": (i32, [P, P, PF64, PF64]),
+    "aleph_hip_batch_run": (i32, [P, P]),
+    "aleph_hip_batch_run_timed": (i32, [P, P, PF64]),
+    "aleph_hip_batch_expect_maxcut": (i32, [P, P, PU64, PU32, PF64]),
+    "aleph_hip_batch_read_state": (i32, [P, P, u32, PF64]),
+    "aleph_hip_batch_stats_get": (i32, [P, C.POINTER(BatchStats)]),
+    "aleph_hip_batch_autotune": (i32, [P, P, u32, C.POINTER(BatchTune), u32, PU32]),
+    "aleph_hip_batch_free": (None, [P]),
+    "aleph_hip_batch_build_host_qaoa": (i32, [u32, PU32, PU64, PU32, u32, PF64, PF64, u32, C.POINTER(BatchOpts),
+                                              C.POINTER(P), C.POINTER(Error)]),
+    "aleph_hip_batch_view_get": (i32, [P, C.POINTER(BatchView)]),
 }
 
 _lib = None

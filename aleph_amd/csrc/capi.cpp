@@ -13,6 +13,7 @@
 #include <memory>
 #include <vector>
 
+#include "batch.hpp"
 #include "common.hpp"
 #include "engine.hpp"
 #include "gates.hpp"
@@ -40,6 +41,10 @@ struct aleph_hip_state {
 
 struct aleph_hip_plan {
   std::unique_ptr<Plan> plan;
+};
+
+struct aleph_hip_batch {
+  std::unique_ptr<Batch> b;
 };
 
 namespace {
@@ -795,6 +800,350 @@ aleph_hip_status aleph_hip_probabilities(aleph_hip_ctx* c, aleph_hip_state* st, 
     ALEPH_HIP_CHECK(hipMemcpyAsync(out, d.p, bins * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
   });
+}
+
+}  // extern "C"
+
+// ---- batches ----
+
+namespace {
+
+BatchConfig batch_config(const aleph_hip_batch_opts* o) {
+  BatchConfig c;
+  if (!o) return c;
+  need(o->mode == ALEPH_HIP_BATCH_PERSISTENT || o->mode == ALEPH_HIP_BATCH_ROUNDS, "unknown batch mode");
+  c.mode = o->mode;
+  c.local_qubits = o->local_qubits;
+  c.chunks_per_item = o->chunks_per_item;
+  c.groups_per_cu = o->groups_per_cu;
+  c.max_ops = o->max_ops_per_pass;
+  c.spin_limit = o->spin_limit;
+  c.nontemporal = (o->flags & ALEPH_HIP_BATCH_NONTEMPORAL) != 0;
+  c.unreachable_deps = (o->flags & ALEPH_HIP_BATCH_UNREACHABLE_DEPS) != 0;
+  return c;
+}
+
+void check_batch_sizes(const uint32_t* nq, uint32_t n_inst) {
+  need(n_inst > 0 && nq != nullptr, "empty batch");
+  for (uint32_t i = 0; i < n_inst; i++) {
+    if (nq[i] < 5 || nq[i] > 32) {
+      Failure f = make_failure(ALEPH_HIP_ERR_TOO_MANY_QUBITS, "batch", "batch instances need 5..32 qubits");
+      f.got = nq[i];
+      f.expected = 32;
+      throw f;
+    }
+  }
+}
+
+void finish_batch(aleph_hip_ctx* c, std::unique_ptr<Batch>& b, aleph_hip_batch** out) {
+  b->plan();
+  b->upload(c->stream, c->device);
+  *out = new aleph_hip_batch{std::move(b)};
+}
+
+template <typename F>
+void time_on_stream(hipStream_t s, double* ms, F&& body) {
+  hipEvent_t e0, e1;
+  ALEPH_HIP_CHECK(hipEventCreate(&e0));
+  ALEPH_HIP_CHECK(hipEventCreate(&e1));
+  ALEPH_HIP_CHECK(hipEventRecord(e0, s));
+  body();
+  ALEPH_HIP_CHECK(hipEventRecord(e1, s));
+  ALEPH_HIP_CHECK(hipEventSynchronize(e1));
+  float f = 0.f;
+  ALEPH_HIP_CHECK(hipEventElapsedTime(&f, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = f;
+}
+
+void run_checked(aleph_hip_ctx* c, Batch& b, double* ms) {
+  if (ms) time_on_stream(c->stream, ms, [&] { b.run(c->stream); });
+  else b.run(c->stream);
+  if (b.failed(c->stream))
+    throw make_failure(ALEPH_HIP_ERR_DEVICE_FAILURE, "batch",
+                       "persistent batch: a dependency wait exceeded its spin limit; results are invalid");
+}
+
+}  // namespace
+
+extern "C" {
+
+aleph_hip_status aleph_hip_batch_create(aleph_hip_ctx* c, uint32_t n_inst, const uint32_t* nq, uint32_t precision,
+                                        const uint32_t* words, const uint64_t* word_offsets, const double* reals,
+                                        const uint64_t* real_offsets, const aleph_hip_batch_opts* opts,
+                                        aleph_hip_batch** out) {
+  return guarded(c, [&] {
+    need(out != nullptr && word_offsets != nullptr && real_offsets != nullptr, "null argument");
+    need(precision == ALEPH_HIP_F64 || precision == ALEPH_HIP_F32, "precision must be 64 or 32");
+    check_batch_sizes(nq, n_inst);
+    auto b = std::make_unique<Batch>();
+    b->precision = precision;
+    b->cfg = batch_config(opts);
+    b->nq.assign(nq, nq + n_inst);
+    b->progs.resize(n_inst);
+    for (uint32_t i = 0; i < n_inst; i++) {
+      need(word_offsets[i] <= word_offsets[i + 1] && real_offsets[i] <= real_offsets[i + 1], "bad offsets");
+      b->progs[i] = decode_program(words + word_offsets[i], word_offsets[i + 1] - word_offsets[i],
+                                   reals + real_offsets[i], real_offsets[i + 1] - real_offsets[i], nq[i]);
+    }
+    finish_batch(c, b, out);
+  });
+}
+
+aleph_hip_status aleph_hip_batch_create_qaoa(aleph_hip_ctx* c, uint32_t n_inst, const uint32_t* nq,
+                                             const uint64_t* edge_offsets, const uint32_t* edges, uint32_t p,
+                                             const double* gammas, const double* betas, uint32_t precision,
+                                             const aleph_hip_batch_opts* opts, aleph_hip_batch** out) {
+  return guarded(c, [&] {
+    need(out != nullptr && edge_offsets != nullptr && edges != nullptr, "null argument");
+    need(p == 0 || (gammas != nullptr && betas != nullptr), "null angles");
+    need(precision == ALEPH_HIP_F64 || precision == ALEPH_HIP_F32, "precision must be 64 or 32");
+    check_batch_sizes(nq, n_inst);
+    auto b = std::make_unique<Batch>();
+    b->precision = precision;
+    b->cfg = batch_config(opts);
+    b->nq.assign(nq, nq + n_inst);
+    b->qaoa.resize(n_inst);
+    for (uint32_t i = 0; i < n_inst; i++) {
+      need(edge_offsets[i] <= edge_offsets[i + 1], "bad edge offsets");
+      QaoaSpec& q = b->qaoa[i];
+      q.edges.assign(edges + 2 * edge_offsets[i], edges + 2 * edge_offsets[i + 1]);
+      (void)cut_desc(nq[i], q.edges.data(), (uint32_t)(q.edges.size() / 2));  // validates the edges
+      q.gammas.assign(gammas + (size_t)i * p, gammas + (size_t)(i + 1) * p);
+      q.betas.assign(betas + (size_t)i * p, betas + (size_t)(i + 1) * p);
+      for (uint32_t l = 0; l < p; l++)
+        if (!std::isfinite(q.gammas[l]) || !std::isfinite(q.betas[l]))
+          throw make_failure(ALEPH_HIP_ERR_NON_FINITE_PARAM, "qaoa", "non-finite angle");
+    }
+    finish_batch(c, b, out);
+  });
+}
+
+aleph_hip_status aleph_hip_batch_set_qaoa_angles(aleph_hip_ctx* c, aleph_hip_batch* h, const double* gammas,
+                                                 const double* betas) {
+  return guarded(c, [&] {
+    need(h != nullptr && h->b && !h->b->qaoa.empty(), "not a QAOA batch");
+    Batch& b = *h->b;
+    for (uint32_t i = 0; i < b.size(); i++) {
+      QaoaSpec& q = b.qaoa[i];
+      const size_t p = q.gammas.size();
+      for (size_t l = 0; l < p; l++) {
+        if (!std::isfinite(gammas[i * p + l]) || !std::isfinite(betas[i * p + l]))
+          throw make_failure(ALEPH_HIP_ERR_NON_FINITE_PARAM, "qaoa", "non-finite angle");
+        q.gammas[l] = gammas[i * p + l];
+        q.betas[l] = betas[i * p + l];
+      }
+    }
+    const size_t n_items = b.items.size(), n_rounds = b.round_begin.size(), n_pass = b.passes.size();
+    b.plan();
+    if (b.items.size() != n_items || b.round_begin.size() != n_rounds || b.passes.size() != n_pass) b.drop_graph();
+    b.upload(c->stream, c->device);
+  });
+}
+
+aleph_hip_status aleph_hip_batch_run(aleph_hip_ctx* c, aleph_hip_batch* h) {
+  return guarded(c, [&] {
+    need(h != nullptr && h->b, "null batch");
+    run_checked(c, *h->b, nullptr);
+  });
+}
+
+aleph_hip_status aleph_hip_batch_run_timed(aleph_hip_ctx* c, aleph_hip_batch* h, double* ms) {
+  return guarded(c, [&] {
+    need(h != nullptr && h->b && ms != nullptr, "null argument");
+    run_checked(c, *h->b, ms);
+  });
+}
+
+aleph_hip_status aleph_hip_batch_expect_maxcut(aleph_hip_ctx* c, aleph_hip_batch* h, const uint64_t* edge_offsets,
+                                               const uint32_t* edges, double* out) {
+  return guarded(c, [&] {
+    need(h != nullptr && h->b && edge_offsets != nullptr && edges != nullptr && out != nullptr, "null argument");
+    Batch& b = *h->b;
+    const uint32_t N = b.size();
+    std::vector<CutDesc> descs(N);
+    uint32_t blocks = 0;
+    for (uint32_t i = 0; i < N; i++) {
+      need(edge_offsets[i] <= edge_offsets[i + 1], "bad edge offsets");
+      descs[i] = cut_desc(b.nq[i], edges + 2 * edge_offsets[i], (uint32_t)(edge_offsets[i + 1] - edge_offsets[i]));
+      descs[i].state_off = b.offs[i];
+      descs[i].block_begin = blocks;
+      blocks += b.nq[i] > kBatchCutBlockBits ? 1u << (b.nq[i] - kBatchCutBlockBits) : 1u;
+    }
+    DevBuf d_desc(N * sizeof(CutDesc));
+    DevBuf d_part((size_t)blocks * sizeof(double));
+    DevBuf d_out(N * sizeof(double));
+    ALEPH_HIP_CHECK(hipMemcpyAsync(d_desc.p, descs.data(), N * sizeof(CutDesc), hipMemcpyHostToDevice, c->stream));
+    if (b.precision == ALEPH_HIP_F64)
+      launch_batch_cut<double>(c->stream, b.pool, (const CutDesc*)d_desc.p, N, blocks, (double*)d_part.p,
+                               (double*)d_out.p);
+    else
+      launch_batch_cut<float>(c->stream, b.pool, (const CutDesc*)d_desc.p, N, blocks, (double*)d_part.p,
+                              (double*)d_out.p);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(out, d_out.p, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+aleph_hip_status aleph_hip_batch_read_state(aleph_hip_ctx* c, aleph_hip_batch* h, uint32_t i, double* out) {
+  return guarded(c, [&] {
+    need(h != nullptr && h->b && out != nullptr, "null argument");
+    Batch& b = *h->b;
+    need(i < b.size(), "instance out of range");
+    const uint64_t amps = 1ull << b.nq[i];
+    if (b.precision == ALEPH_HIP_F64) {
+      ALEPH_HIP_CHECK(hipMemcpyAsync(out, (const char*)b.pool + b.offs[i] * 16, amps * 16, hipMemcpyDeviceToHost,
+                                     c->stream));
+      ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    } else {
+      std::vector<float> f(2 * amps);
+      ALEPH_HIP_CHECK(hipMemcpyAsync(f.data(), (const char*)b.pool + b.offs[i] * 8, amps * 8, hipMemcpyDeviceToHost,
+                                     c->stream));
+      ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+      for (size_t j = 0; j < 2 * amps; j++) out[j] = f[j];
+    }
+  });
+}
+
+aleph_hip_status aleph_hip_batch_stats_get(const aleph_hip_batch* h, aleph_hip_batch_stats* out) {
+  if (!h || !h->b || !out) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  const Batch& b = *h->b;
+  aleph_hip_batch_stats s{};
+  s.plan_ms = b.plan_ms;
+  s.sweep_bytes = b.sweep_bytes;
+  s.pool_bytes = b.pool_bytes;
+  s.n_instances = b.size();
+  s.n_passes = b.total_passes;
+  s.n_items = (uint32_t)b.items.size();
+  s.n_rounds = b.round_begin.empty() ? 0 : (uint32_t)b.round_begin.size() - 1;
+  s.grid = b.grid;
+  s.threads = b.launch.threads;
+  s.lds_bytes = (uint32_t)b.launch.lds_bytes;
+  s.mode = b.cfg.mode;
+  s.local_qubits = b.cfg.local_qubits;
+  s.chunks_per_item = b.cfg.chunks_per_item;
+  s.groups_per_cu = b.cfg.groups_per_cu;
+  *out = s;
+  return ALEPH_HIP_OK;
+}
+
+aleph_hip_status aleph_hip_batch_autotune(aleph_hip_ctx* c, aleph_hip_batch* h, uint32_t reps,
+                                          aleph_hip_batch_tune* results, uint32_t max_results, uint32_t* n_results) {
+  return guarded(c, [&] {
+    need(h != nullptr && h->b, "null batch");
+    Batch& b = *h->b;
+    reps = std::max(1u, reps);
+    const BatchConfig base = b.cfg;
+    const bool f32 = b.precision == ALEPH_HIP_F32;
+    const uint32_t ms_default = f32 ? 13u : 12u;
+    std::vector<uint32_t> chunk_bits = {ms_default - 2, ms_default - 1, ms_default};
+    const uint32_t cpis[] = {4, 16, 64};
+    const uint32_t modes[] = {ALEPH_HIP_BATCH_PERSISTENT, ALEPH_HIP_BATCH_ROUNDS};
+    std::vector<aleph_hip_batch_tune> all;
+    aleph_hip_batch_tune best{};
+    best.ms = 1e300;
+    for (uint32_t m : chunk_bits) {
+      for (uint32_t cpi : cpis) {
+        for (uint32_t mode : modes) {
+          b.cfg = base;
+          b.cfg.local_qubits = m;
+          b.cfg.chunks_per_item = cpi;
+          b.cfg.mode = mode;
+          b.drop_graph();
+          b.plan();
+          b.upload(c->stream, c->device);
+          run_checked(c, b, nullptr);  // warm-up, also captures the graph
+          double t = 1e300;
+          for (uint32_t r = 0; r < reps; r++) {
+            double ms = 0.0;
+            run_checked(c, b, &ms);
+            t = std::min(t, ms);
+          }
+          aleph_hip_batch_tune rec{mode, m, cpi, b.cfg.groups_per_cu, t};
+          all.push_back(rec);
+          if (t < best.ms) best = rec;
+        }
+      }
+    }
+    b.cfg = base;
+    b.cfg.mode = best.mode;
+    b.cfg.local_qubits = best.local_qubits;
+    b.cfg.chunks_per_item = best.chunks_per_item;
+    b.drop_graph();
+    b.plan();
+    b.upload(c->stream, c->device);
+    if (n_results) *n_results = (uint32_t)all.size();
+    if (results)
+      for (uint32_t i = 0; i < std::min<uint32_t>(max_results, (uint32_t)all.size()); i++) results[i] = all[i];
+  });
+}
+
+void aleph_hip_batch_free(aleph_hip_batch* h) { delete h; }
+
+aleph_hip_status aleph_hip_batch_build_host_qaoa(uint32_t n_inst, const uint32_t* nq, const uint64_t* edge_offsets,
+                                                 const uint32_t* edges, uint32_t p, const double* gammas,
+                                                 const double* betas, uint32_t precision,
+                                                 const aleph_hip_batch_opts* opts, aleph_hip_batch** out,
+                                                 aleph_hip_error* err) {
+  aleph_hip_ctx tmp{};
+  const aleph_hip_status st = guarded(nullptr, [&] {
+    need(out != nullptr && edge_offsets != nullptr && edges != nullptr, "null argument");
+    need(p == 0 || (gammas != nullptr && betas != nullptr), "null angles");
+    need(precision == ALEPH_HIP_F64 || precision == ALEPH_HIP_F32, "precision must be 64 or 32");
+    check_batch_sizes(nq, n_inst);
+    auto b = std::make_unique<Batch>();
+    b->precision = precision;
+    b->cfg = batch_config(opts);
+    b->nq.assign(nq, nq + n_inst);
+    b->qaoa.resize(n_inst);
+    for (uint32_t i = 0; i < n_inst; i++) {
+      QaoaSpec& q = b->qaoa[i];
+      q.edges.assign(edges + 2 * edge_offsets[i], edges + 2 * edge_offsets[i + 1]);
+      (void)cut_desc(nq[i], q.edges.data(), (uint32_t)(q.edges.size() / 2));
+      q.gammas.assign(gammas + (size_t)i * p, gammas + (size_t)(i + 1) * p);
+      q.betas.assign(betas + (size_t)i * p, betas + (size_t)(i + 1) * p);
+    }
+    try {
+      b->plan();
+    } catch (const Failure& f) {
+      set_error(&tmp, f);
+      throw;
+    }
+    *out = new aleph_hip_batch{std::move(b)};
+  });
+  if (err) *err = tmp.err;
+  return st;
+}
+
+aleph_hip_status aleph_hip_batch_view_get(const aleph_hip_batch* h, aleph_hip_batch_view* out) {
+  if (!h || !h->b || !out) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  const Batch& b = *h->b;
+  aleph_hip_batch_view v{};
+  v.items = b.items.data();
+  v.n_items = (uint32_t)b.items.size();
+  v.item_bytes = sizeof(BatchItem);
+  v.passes = b.passes.data();
+  v.n_passes = (uint32_t)b.passes.size();
+  v.pass_bytes = sizeof(PassDev);
+  v.stages = b.stages.data();
+  v.n_stages = (uint32_t)b.stages.size();
+  v.stage_bytes = sizeof(StageDev);
+  v.ops = b.ops.data();
+  v.n_ops = (uint32_t)b.ops.size();
+  v.op_bytes = sizeof(DevOp);
+  v.terms = b.terms.data();
+  v.n_terms = (uint32_t)b.terms.size();
+  v.term_bytes = sizeof(PhaseTerm);
+  v.offsets = b.offs.data();
+  v.n_instances = b.size();
+  v.n_rounds = b.round_begin.empty() ? 0 : (uint32_t)b.round_begin.size() - 1;
+  v.round_begin = b.round_begin.data();
+  v.threads = b.launch.threads;
+  v.lds_bytes = (uint32_t)b.launch.lds_bytes;
+  v.total_amps = b.total_amps;
+  *out = v;
+  return ALEPH_HIP_OK;
 }
 
 }  // extern "C"

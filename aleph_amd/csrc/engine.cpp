@@ -565,6 +565,16 @@ uint32_t phase_class(uint32_t rm, uint32_t G, bool* general) {
   return 0;
 }
 
+// FP32 plans keep op matrices as floats (the first `count` slots of the
+// matrix area), so the kernel reads them without conversion.
+void narrow_matrix(DevOp& op, int count, uint32_t precision) {
+  if (precision != ALEPH_HIP_F32) return;
+  float f[32];
+  for (int i = 0; i < count; i++) f[i] = (float)op.m[i];
+  std::memset(op.m, 0, sizeof(op.m));
+  std::memcpy(op.m, f, (size_t)count * sizeof(float));
+}
+
 void swap_2q_matrix(double* m) {
   // index (a b) -> (b a): swap bit 0 and bit 1 of rows and columns
   double t[32];
@@ -745,6 +755,7 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
           op.a = reg_of_local[local_of[g->qa]];
           op.cmask = g->cmask;
           std::memcpy(op.m, g->m, 8 * sizeof(double));
+          narrow_matrix(op, 8, p.precision);
           p.ops.push_back(op);
           break;
         case LGate::G2Q: {
@@ -758,6 +769,7 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
           op.a = ra;
           op.b = rb;
           op.cmask = g->cmask;
+          narrow_matrix(op, 32, p.precision);
           p.ops.push_back(op);
           break;
         }
@@ -773,6 +785,7 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
           op.mb = g->n_diag_targets == 2 ? (1ull << g->qb) : 0;
           op.cmask = g->cmask;
           std::memcpy(op.m, g->m, 8 * sizeof(double));
+          narrow_matrix(op, 8, p.precision);
           p.ops.push_back(op);
           break;
         case LGate::TERMS:

@@ -1,0 +1,374 @@
+// Device-side body of the fused LDS-staged pass, shared by the one-pass
+// launcher (k_fused.hip) and the batched executors (k_batch.hip).
+//
+// One chunk of 2^m amplitudes (the index bits outside the pass's chunk
+// qubits fixed by the chunk number) is read from HBM once; every op of the
+// pass is applied while it sits in registers (2^G amplitudes per thread) or
+// is re-distributed through LDS between stages; it is written back once.
+#pragma once
+
+#include <type_traits>
+
+#include "common.hpp"
+#include "plan.hpp"
+
+namespace aleph {
+namespace fused {
+
+
+// XOR-fold swizzle of the chunk-local index into LDS slots: bits >= SW are
+// folded into the low SW bits, so lanes whose thread bits map to high local
+// bits still spread over distinct 128-byte bank windows.
+template <int SW>
+__device__ __forceinline__ uint32_t swz(uint32_t li) {
+  const uint32_t h = li >> SW;
+  return li ^ ((h ^ (h >> SW) ^ (h >> (2 * SW)) ^ (h >> (3 * SW))) & ((1u << SW) - 1u));
+}
+
+template <typename T>
+__device__ __forceinline__ C2<T> cmac2(T ar, T ai, C2<T> x, T br, T bi, C2<T> y) {
+  C2<T> o;
+  o.x = ar * x.x - ai * x.y + br * y.x - bi * y.y;
+  o.y = ar * x.y + ai * x.x + br * y.y + bi * y.x;
+  return o;
+}
+
+// Uniform global offset of register slot r (bit g of r selects rg[g]).
+template <int G>
+__device__ __forceinline__ uint64_t roff(const uint64_t (&rg)[G], int r) {
+  uint64_t o = 0;
+#pragma unroll
+  for (int g = 0; g < G; g++)
+    if (r & (1 << g)) o |= rg[g];
+  return o;
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t loff(const uint32_t (&rl)[G], int r) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int g = 0; g < G; g++)
+    if (r & (1 << g)) o |= rl[g];
+  return o;
+}
+
+template <typename T, int G, int B>
+__device__ __forceinline__ void op_1q(C2<T> (&a)[1 << G], const DevOp* __restrict__ op, uint64_t tgo,
+                                      const uint64_t (&rg)[G]) {
+  const T* mm = reinterpret_cast<const T*>(op->m);  // matrix stored in the plan's precision
+  const T m00r = mm[0], m00i = mm[1], m01r = mm[2], m01i = mm[3];
+  const T m10r = mm[4], m10i = mm[5], m11r = mm[6], m11i = mm[7];
+  const uint64_t cm = op->cmask;
+  if (cm == 0) {
+#pragma unroll
+    for (int r = 0; r < (1 << G); r++) {
+      if (r & (1 << B)) continue;
+      const int s = r | (1 << B);
+      const C2<T> x = a[r], y = a[s];
+      a[r] = cmac2(m00r, m00i, x, m01r, m01i, y);
+      a[s] = cmac2(m10r, m10i, x, m11r, m11i, y);
+    }
+  } else {
+    const uint64_t tc = tgo & cm;
+#pragma unroll
+    for (int r = 0; r < (1 << G); r++) {
+      if (r & (1 << B)) continue;
+      const int s = r | (1 << B);
+      const C2<T> x = a[r], y = a[s];
+      const bool on = (tc | (roff<G>(rg, r) & cm)) == cm;
+      const C2<T> nx = cmac2(m00r, m00i, x, m01r, m01i, y);
+      const C2<T> ny = cmac2(m10r, m10i, x, m11r, m11i, y);
+      a[r] = on ? nx : x;
+      a[s] = on ? ny : y;
+    }
+  }
+}
+
+// Insert a zero bit at position p of x (compile time).
+constexpr int ins0(int x, int p) { return (x & ((1 << p) - 1)) | ((x >> p) << (p + 1)); }
+
+// One quartet (register bits A > B cleared in slot Q): its four inputs are
+// copied out, then each output row is formed from one matrix row read just
+// before use.  For FP64 the empty asm with a memory clobber keeps the
+// compiler from hoisting all sixteen matrix entries into registers at once.
+template <typename T, int G, int A, int B, int Q, bool CTL>
+__device__ __forceinline__ void quartets_2q(C2<T> (&a)[1 << G], const DevOp* __restrict__ op,
+                                            const bool (&on)[1 << (G - 2)]) {
+  if constexpr (Q < (1 << (G - 2))) {
+    constexpr int r = ins0(ins0(Q, B), A);
+    constexpr int idx[4] = {r, r | (1 << B), r | (1 << A), r | (1 << A) | (1 << B)};
+    const C2<T> v0 = a[idx[0]], v1 = a[idx[1]], v2 = a[idx[2]], v3 = a[idx[3]];
+    C2<T> out[4];
+#pragma unroll
+    for (int row = 0; row < 4; row++) {
+      if constexpr (sizeof(T) == 8) asm volatile("" ::: "memory");
+      const T* mrow = reinterpret_cast<const T*>(op->m) + 8 * row;
+      const T m0r = mrow[0], m0i = mrow[1], m1r = mrow[2], m1i = mrow[3];
+      const T m2r = mrow[4], m2i = mrow[5], m3r = mrow[6], m3i = mrow[7];
+      T re = m0r * v0.x;
+      T im = m0r * v0.y;
+      re = fma(-m0i, v0.y, re);
+      im = fma(m0i, v0.x, im);
+      re = fma(m1r, v1.x, re);
+      re = fma(-m1i, v1.y, re);
+      im = fma(m1r, v1.y, im);
+      im = fma(m1i, v1.x, im);
+      re = fma(m2r, v2.x, re);
+      re = fma(-m2i, v2.y, re);
+      im = fma(m2r, v2.y, im);
+      im = fma(m2i, v2.x, im);
+      re = fma(m3r, v3.x, re);
+      re = fma(-m3i, v3.y, re);
+      im = fma(m3r, v3.y, im);
+      im = fma(m3i, v3.x, im);
+      out[row].x = re;
+      out[row].y = im;
+    }
+    if constexpr (CTL) {
+      const bool live = on[Q];
+      a[idx[0]] = live ? out[0] : v0;
+      a[idx[1]] = live ? out[1] : v1;
+      a[idx[2]] = live ? out[2] : v2;
+      a[idx[3]] = live ? out[3] : v3;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) a[idx[k]] = out[k];
+    }
+    quartets_2q<T, G, A, B, Q + 1, CTL>(a, op, on);
+  }
+}
+
+template <typename T, int G, int A, int B>  // A = matrix MSB register bit, B = LSB, A > B
+__device__ __forceinline__ void op_2q(C2<T> (&a)[1 << G], const DevOp* __restrict__ op, uint64_t tgo,
+                                      const uint64_t (&rg)[G]) {
+  const uint64_t cm = op->cmask;
+  bool on[1 << (G - 2)];
+  if (cm == 0) {
+    quartets_2q<T, G, A, B, 0, false>(a, op, on);
+    return;
+  }
+  const uint64_t tc = tgo & cm;
+#pragma unroll
+  for (int q = 0; q < (1 << (G - 2)); q++) on[q] = (tc | (roff<G>(rg, ins0(ins0(q, B), A)) & cm)) == cm;
+  quartets_2q<T, G, A, B, 0, true>(a, op, on);
+}
+
+template <typename T, int G>
+__device__ __forceinline__ void op_diag(C2<T> (&a)[1 << G], const DevOp* __restrict__ op, uint64_t tgo,
+                                        const uint64_t (&rg)[G]) {
+  const uint64_t ma = op->ma, mb = op->mb, cm = op->cmask;
+  const T* dm = reinterpret_cast<const T*>(op->m);
+  const T d0r = dm[0], d0i = dm[1], d1r = dm[2], d1i = dm[3];
+  const T d2r = dm[4], d2i = dm[5], d3r = dm[6], d3i = dm[7];
+#pragma unroll
+  for (int r = 0; r < (1 << G); r++) {
+    const uint64_t idx = tgo | roff<G>(rg, r);
+    const bool hi = (idx & ma) != 0, lo = (idx & mb) != 0;
+    const T dr = hi ? (lo ? d3r : d2r) : (lo ? d1r : d0r);
+    const T di = hi ? (lo ? d3i : d2i) : (lo ? d1i : d0i);
+    const C2<T> x = a[r];
+    C2<T> y;
+    y.x = x.x * dr - x.y * di;
+    y.y = x.x * di + x.y * dr;
+    a[r] = ((idx & cm) == cm) ? y : x;
+  }
+}
+
+template <typename T, int G>
+__device__ __forceinline__ void op_sign(C2<T> (&a)[1 << G], const DevOp* __restrict__ op, uint64_t tgo) {
+  const uint64_t mo = op->cmask;
+  const uint32_t rm = op->a;
+  const bool th = (tgo & mo) == mo;
+#pragma unroll
+  for (int r = 0; r < (1 << G); r++) {
+    const bool flip = th && ((uint32_t)r & rm) == rm;
+    a[r].x = flip ? -a[r].x : a[r].x;
+    a[r].y = flip ? -a[r].y : a[r].y;
+  }
+}
+
+// Sum of the angles of terms [b, e) that fire for this thread.  The loads
+// are independent, so unrolling keeps several scalar loads in flight.
+__device__ __forceinline__ double sum_terms(const PhaseTerm* __restrict__ terms, uint32_t b, uint32_t e,
+                                            uint64_t tgo) {
+  double s = 0.0;
+#pragma unroll 4
+  for (uint32_t t = b; t < e; t++) {
+    const uint64_t mo = terms[t].mask_out;
+    s += ((tgo & mo) == mo) ? terms[t].angle : 0.0;
+  }
+  return s;
+}
+
+template <typename T, int G>
+__device__ __forceinline__ void op_phase(C2<T> (&a)[1 << G], const DevOp* __restrict__ op,
+                                         const PhaseTerm* __restrict__ terms, uint64_t tgo) {
+  // Terms on at most two register bits fold into 1 + G + G(G-1)/2
+  // accumulators (PhaseSlots).  The host sorts them by accumulator and
+  // stores the class boundaries (relative to term_off) in the op's matrix
+  // slots; the rare terms on three or more register bits follow (op->b of
+  // them) and are evaluated per amplitude.
+  using L = PhaseSlots<G>;
+  const uint32_t* bounds = reinterpret_cast<const uint32_t*>(op->m);
+  double acc[L::kCount];
+#pragma unroll
+  for (int k = 0; k < L::kCount; k++)
+    acc[k] = sum_terms(terms, op->term_off + bounds[k], op->term_off + bounds[k + 1], tgo);
+  const uint32_t end = op->term_off + op->n_terms;
+  const uint32_t gen_end = end + op->b;
+#pragma unroll
+  for (int r = 0; r < (1 << G); r++) {
+    double phi = acc[0];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      if (r & (1 << g)) phi += acc[L::linear(g)];
+#pragma unroll
+      for (int h = g + 1; h < G; h++)
+        if ((r & (1 << g)) && (r & (1 << h))) phi += acc[L::pair(g, h)];
+    }
+    for (uint32_t t = end; t < gen_end; t++) {
+      const PhaseTerm* pt = terms + t;
+      const uint64_t mo = pt->mask_out;
+      const uint32_t rm = pt->rmask;
+      if ((tgo & mo) == mo && ((uint32_t)r & rm) == rm) phi += pt->angle;
+    }
+    double s, c;
+    sincos(phi, &s, &c);
+    const C2<T> x = a[r];
+    a[r].x = x.x * (T)c - x.y * (T)s;
+    a[r].y = x.x * (T)s + x.y * (T)c;
+  }
+}
+
+template <typename T, int G, int B>
+__device__ __forceinline__ void dispatch_1q(C2<T> (&a)[1 << G], const DevOp* op, uint64_t tgo,
+                                            const uint64_t (&rg)[G], uint32_t b) {
+  if constexpr (B < G) {
+    if (b == (uint32_t)B) op_1q<T, G, B>(a, op, tgo, rg);
+    else dispatch_1q<T, G, B + 1>(a, op, tgo, rg, b);
+  }
+}
+
+template <typename T, int G, int A, int B>
+__device__ __forceinline__ void dispatch_2q(C2<T> (&a)[1 << G], const DevOp* op, uint64_t tgo,
+                                            const uint64_t (&rg)[G], uint32_t ab) {
+  if constexpr (A < G) {
+    if constexpr (B < A) {
+      if (ab == (uint32_t)(A * 8 + B)) op_2q<T, G, A, B>(a, op, tgo, rg);
+      else dispatch_2q<T, G, A, B + 1>(a, op, tgo, rg, ab);
+    } else {
+      dispatch_2q<T, G, A + 1, 0>(a, op, tgo, rg, ab);
+    }
+  }
+}
+
+template <typename T, int G, int FEAT>
+__device__ __forceinline__ void run_ops(C2<T> (&a)[1 << G], const DevOp* __restrict__ ops,
+                                        const PhaseTerm* __restrict__ terms, uint32_t begin, uint32_t end,
+                                        uint64_t tgo, const uint64_t (&rg)[G]) {
+  for (uint32_t o = begin; o < end; o++) {
+    const DevOp* op = ops + o;
+    const uint32_t type = op->type;
+    if (type == OP_1Q) {
+      dispatch_1q<T, G, 0>(a, op, tgo, rg, op->a);
+    } else if (type == OP_SIGN) {
+      op_sign<T, G>(a, op, tgo);
+    } else if (type == OP_DIAG) {
+      op_diag<T, G>(a, op, tgo, rg);
+    } else if (type == OP_2Q) {
+      if constexpr ((FEAT & kFeat2Q) != 0) dispatch_2q<T, G, 1, 0>(a, op, tgo, rg, op->a * 8 + op->b);
+    } else {
+      if constexpr ((FEAT & kFeatPhase) != 0) op_phase<T, G>(a, op, terms, tgo);
+    }
+  }
+}
+
+typedef double v2f64 __attribute__((ext_vector_type(2)));
+typedef float v2f32 __attribute__((ext_vector_type(2)));
+
+template <bool NT, typename V>
+__device__ __forceinline__ V gload(const V* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
+template <bool NT, typename V>
+__device__ __forceinline__ void gstore(V* p, V v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+// Process chunk `chunk` of pass `pd` on `state`.  With GUARD, threads with
+// tid >= 2^(m - G) hold no amplitudes but take part in every barrier, so a
+// workgroup sized for the largest chunk can run smaller passes; without it
+// the workgroup has exactly 2^(m - G) threads.
+template <typename T, int G, int FEAT, bool NT, bool GUARD>
+__device__ __forceinline__ void fused_chunk(C2<T>* __restrict__ state, const PassDev* __restrict__ pd,
+                                            const StageDev* __restrict__ stages, const DevOp* __restrict__ ops,
+                                            const PhaseTerm* __restrict__ terms, uint64_t chunk,
+                                            C2<T>* __restrict__ lds) {
+  constexpr int SW = sizeof(T) == 8 ? 3 : 4;
+  constexpr int N = 1 << G;
+  using V = typename std::conditional<sizeof(T) == 8, v2f64, v2f32>::type;
+  const uint32_t m = pd->m;
+  uint64_t base = chunk;
+  for (uint32_t i = 0; i < m; i++) base = insert_zero(base, pd->local_pos[i]);
+  const uint32_t tid = threadIdx.x;
+  const bool active = !GUARD || tid < (1u << (m - G));
+  const uint32_t ns = pd->n_stages;
+  C2<T> a[N];
+  for (uint32_t s = 0; s < ns; s++) {
+    const StageDev* sd = stages + pd->stage_begin + s;
+    uint64_t tgo = base;
+    uint32_t tlo = 0;
+    const uint32_t ntb = sd->n_thread_bits;
+    for (uint32_t j = 0; j < ntb; j++) {
+      if ((tid >> j) & 1u) {
+        tgo |= sd->tg[j];
+        tlo |= sd->tl[j];
+      }
+    }
+    uint64_t rg[G];
+    uint32_t rl[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      rg[g] = sd->rg[g];
+      rl[g] = sd->rl[g];
+    }
+    if (active) {
+      if (s == 0) {
+#pragma unroll
+        for (int r = 0; r < N; r++) {
+          const V v = gload<NT>(reinterpret_cast<const V*>(state + (tgo | roff<G>(rg, r))));
+          a[r].x = v.x;
+          a[r].y = v.y;
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < N; r++) a[r] = lds[swz<SW>(tlo | loff<G>(rl, r))];
+      }
+      run_ops<T, G, FEAT>(a, ops, terms, sd->op_begin, sd->op_end, tgo, rg);
+    }
+    if (s + 1 == ns) {
+      if (active) {
+#pragma unroll
+        for (int r = 0; r < N; r++) {
+          V v;
+          v.x = a[r].x;
+          v.y = a[r].y;
+          gstore<NT>(reinterpret_cast<V*>(state + (tgo | roff<G>(rg, r))), v);
+        }
+      }
+    } else {
+      if (s > 0) __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int r = 0; r < N; r++) lds[swz<SW>(tlo | loff<G>(rl, r))] = a[r];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace fused
+}  // namespace aleph

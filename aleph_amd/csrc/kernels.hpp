@@ -43,6 +43,36 @@ void launch_fused_pass(hipStream_t s, void* state, uint32_t n, uint32_t m, uint3
                        uint32_t reg_bits, const PassDev* pd, const StageDev* stages, const DevOp* ops,
                        const PhaseTerm* terms);
 
+// ---- batched plans (k_batch.hip) ----
+struct BatchLaunch {
+  uint32_t threads = 512;      // workgroup size: 2^(m_max - G)
+  size_t lds_bytes = 0;        // 2^m_max amplitudes
+  bool nontemporal = false;
+};
+template <typename T>
+void launch_batch_round(hipStream_t s, const BatchLaunch& cfg, void* pool, const BatchItem* items, uint32_t n_items,
+                        const PassDev* passes, const StageDev* stages, const DevOp* ops, const PhaseTerm* terms);
+template <typename T>
+void launch_batch_persistent(hipStream_t s, const BatchLaunch& cfg, void* pool, const BatchItem* items,
+                             uint32_t n_items, uint32_t grid, const PassDev* passes, const StageDev* stages,
+                             const DevOp* ops, const PhaseTerm* terms, uint32_t* done, BatchCtl* ctl,
+                             uint32_t spin_limit);
+// Resident workgroups per CU of the persistent executor.
+template <typename T> int batch_persistent_occupancy(const BatchLaunch& cfg);
+// Allow the LDS size of `cfg` on both executors (call before graph capture).
+template <typename T> void batch_configure(const BatchLaunch& cfg);
+// Zero the pool, set amplitude 0 of every instance to 1, reset the queue.
+template <typename T>
+void launch_batch_init(hipStream_t s, void* pool, uint64_t n_amps, const uint64_t* offs, uint32_t n_inst,
+                       uint32_t* done, BatchCtl* ctl);
+void launch_batch_reset(hipStream_t s, uint32_t* done, uint32_t n_inst, BatchCtl* ctl);
+// MaxCut expectation per instance: out[i] = sum_x |psi_x|^2 cut_i(x); one
+// partial per 4096-amplitude block (descs[i].block_begin numbers them).
+constexpr uint32_t kBatchCutBlockBits = 12;
+template <typename T>
+void launch_batch_cut(hipStream_t s, const void* pool, const CutDesc* descs, uint32_t n_inst, uint32_t n_blocks,
+                      double* partials, double* out);
+
 // ---- readout (k_readout.hip) ----
 // Result layout in `out` (device, 3 doubles): [sum of |a|^2 over bit q clear,
 // sum over bit q set, non-finite count]; qubit < 0 puts everything in [0].  `partials` needs 3 * kReduceBlocks doubles.

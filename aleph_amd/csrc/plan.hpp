@@ -87,4 +87,38 @@ struct alignas(16) PassDev {
   uint32_t pad2;
 };
 
+// ---- batched execution of many small plans ----
+
+// One unit of work of a batch: chunks [chunk_begin, chunk_begin +
+// chunk_count) of one fused pass of one instance.  The instance's done
+// counter (chunks finished over all its passes) must reach dep_target
+// before the item may start; the queue lists every pass's items after the
+// items of the instance's earlier passes.
+struct alignas(16) BatchItem {
+  uint64_t state_off;            // first amplitude of the instance in the pool
+  uint32_t pass;                 // global PassDev index
+  uint32_t inst;
+  uint32_t chunk_begin, chunk_count;
+  uint32_t dep_target;
+  uint32_t pad;
+};
+
+// Queue head and failure flag of the persistent executor.
+struct alignas(16) BatchCtl {
+  uint32_t head;
+  uint32_t error;                // set when a dependency wait exceeded its spin limit
+  uint32_t pad[2];
+};
+
+// MaxCut cost of one instance: adj[u] = neighbour mask of qubit u (n <= 32);
+// ll[r] = cut among edges inside qubits 0..3 for the 16 settings r of
+// those bits, 4 bits each.
+struct alignas(16) CutDesc {
+  uint64_t state_off;
+  uint32_t n;
+  uint32_t block_begin;          // first partial-sum slot of this instance
+  uint64_t ll;
+  uint32_t adj[32];
+};
+
 }  // namespace aleph

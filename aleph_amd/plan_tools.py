@@ -108,3 +108,73 @@ class HostPlan:
                          ops=[[OP_NAMES[self.ops[o].type] for o in range(sd.op_begin, sd.op_end)] for sd in stages])
             out.append(d)
         return out
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("state_off", u64), ("pass_", u32), ("inst", u32), ("chunk_begin", u32), ("chunk_count", u32),
+                ("dep_target", u32), ("pad", u32)]
+
+
+class _Steps:
+    """A minimal plan-like view of one batch instance (its fused passes in order)."""
+
+    def __init__(self, hb: "HostBatch", i: int):
+        self.num_qubits = hb.num_qubits[i]
+        self.passes, self.stages, self.ops, self.terms, self.payload = hb.passes, hb.stages, hb.ops, hb.terms, b""
+        self.steps = []
+        for g in hb.instance_passes[i]:
+            s = Step()
+            s.kind = 0
+            s.pass_ = g
+            self.steps.append(s)
+
+
+class HostBatch:
+    """A QAOA batch planned on the host (no device); packed arrays copied out."""
+
+    def __init__(self, graphs, gammas, betas, precision: int = 32, mode: int = 0, local_qubits: int = 0,
+                 chunks_per_item: int = 0):
+        lib = L.load()
+        nq = np.ascontiguousarray(np.array([g[0] for g in graphs], dtype=np.uint32))
+        offs = np.zeros(len(graphs) + 1, dtype=np.uint64)
+        flat = []
+        for i, (_, edges) in enumerate(graphs):
+            offs[i + 1] = offs[i] + len(edges)
+            for u, v in edges:
+                flat += [u, v]
+        flat = np.ascontiguousarray(np.array(flat or [0], dtype=np.uint32))
+        ga = np.ascontiguousarray(np.asarray(gammas, dtype=np.float64).reshape(len(graphs), -1))
+        be = np.ascontiguousarray(np.asarray(betas, dtype=np.float64).reshape(len(graphs), -1))
+        opts = L.BatchOpts(mode, local_qubits, chunks_per_item, 0, 0, 0, 0, 0)
+        h = C.c_void_p()
+        err = L.Error()
+        code = lib.aleph_hip_batch_build_host_qaoa(len(graphs), nq.ctypes.data_as(L.PU32),
+                                                   offs.ctypes.data_as(L.PU64), flat.ctypes.data_as(L.PU32),
+                                                   ga.shape[1], ga.ctypes.data_as(L.PF64), be.ctypes.data_as(L.PF64),
+                                                   precision, C.byref(opts), C.byref(h), C.byref(err))
+        if code != L.OK:
+            raise RuntimeError(f"batch plan failed with status {code}: {err.message.decode()}")
+        try:
+            v = L.BatchView()
+            lib.aleph_hip_batch_view_get(h, C.byref(v))
+            self.items = [BatchItem.from_buffer_copy(s) for s in _array(v.items, v.n_items, BatchItem, v.item_bytes)]
+            self.passes = [PassDev.from_buffer_copy(s) for s in _array(v.passes, v.n_passes, PassDev, v.pass_bytes)]
+            self.stages = [StageDev.from_buffer_copy(s) for s in _array(v.stages, v.n_stages, StageDev,
+                                                                         v.stage_bytes)]
+            self.ops = [DevOp.from_buffer_copy(s) for s in _array(v.ops, v.n_ops, DevOp, v.op_bytes)]
+            self.terms = [PhaseTerm.from_buffer_copy(s) for s in _array(v.terms, v.n_terms, PhaseTerm,
+                                                                         v.term_bytes)]
+            self.offsets = [int(v.offsets[i]) for i in range(v.n_instances)]
+            self.round_begin = [int(v.round_begin[i]) for i in range(v.n_rounds + 1)]
+            self.threads, self.lds_bytes, self.total_amps = v.threads, v.lds_bytes, v.total_amps
+        finally:
+            lib.aleph_hip_batch_free(h)
+        self.num_qubits = nq.tolist()
+        self.precision = precision
+        self.instance_passes = [[] for _ in graphs]
+        for it in self.items:
+            if not self.instance_passes[it.inst] or self.instance_passes[it.inst][-1] != it.pass_:
+                self.instance_passes[it.inst].append(it.pass_)
+
+    def instance_plan(self, i: int):
+        return _Steps(self, i)

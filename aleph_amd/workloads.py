@@ -14,6 +14,8 @@ seed), not by stored files:
 * `hea(n, layers, seed)`: RY and RZ on every qubit, then a CX ladder.
 * `qaoa_maxcut(n, edges, gammas, betas)`: H^n, then per layer RZZ(γ) on
   every edge (as CNOT·RZ(γ)·CNOT = exp(−iγ Z⊗Z/2)) and RX(β) on every qubit.
+* `qaoa_batch(count, seed)`: seeded QAOA MaxCut instances with n even in
+  [12, 24], random 3-regular graphs and p = 4 layers of angles in [0, π).
 """
 
 from __future__ import annotations
@@ -131,6 +133,23 @@ def qaoa_maxcut(n: int, edges, gammas, betas) -> Circuit:
         for q in range(n):
             c.rx(b, q)
     return c
+
+
+def qaoa_batch(count: int, seed: int, n_min: int = 12, n_max: int = 24, p: int = 4):
+    """Seeded batch of QAOA MaxCut instances: n uniform over the even values in
+    [n_min, n_max], a random 3-regular graph per instance, and gamma, beta
+    uniform in [0, pi) per layer.  Returns (graphs, gammas, betas) with graphs a
+    list of (n, edges) and the angle arrays of shape (count, p)."""
+    rng = SplitMix64(seed)
+    evens = list(range(n_min + (n_min & 1), n_max + 1, 2))
+    graphs, gammas, betas = [], np.zeros((count, p)), np.zeros((count, p))
+    for i in range(count):
+        n = evens[rng.next_u64() % len(evens)]
+        graphs.append((n, random_3_regular(n, rng.next_u64())))
+        for l in range(p):
+            gammas[i, l] = math.pi * rng.uniform()
+            betas[i, l] = math.pi * rng.uniform()
+    return graphs, gammas, betas
 
 
 # ---------------- OpenQASM 3 subset ----------------

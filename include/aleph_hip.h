@@ -252,6 +252,97 @@ aleph_hip_status aleph_hip_expectation(aleph_hip_ctx* ctx, aleph_hip_state* st, 
 aleph_hip_status aleph_hip_probabilities(aleph_hip_ctx* ctx, aleph_hip_state* st, const uint32_t* qubits,
                                          uint32_t n_qubits, double* out);
 
+/* ---- batches: many small circuits of uneven qubit count in one launch ----
+ * No counterpart in the reference, which runs a parameter sweep as one
+ * run_with_outcomes call per circuit (crates/aleph-backend/src/lib.rs:211-248)
+ * and scores MaxCut with expectation_pauli_sum over maxcut_pauli_sum
+ * (lib.rs:422-432, crates/aleph-ir/src/ansatz.rs).  Every instance is planned
+ * into fused passes; the batch runs as one persistent launch (or one launch
+ * per pass round), captured once into a hipGraph and replayed. */
+typedef struct aleph_hip_batch aleph_hip_batch;
+enum { ALEPH_HIP_BATCH_PERSISTENT = 0, ALEPH_HIP_BATCH_ROUNDS = 1 };
+enum {
+  ALEPH_HIP_BATCH_NONTEMPORAL = 1,
+  /* Testing aid: every dependency wait of the persistent executor is made
+   * unsatisfiable, so the bounded wait gives up and the run reports
+   * DeviceFailure (every workgroup still exits). */
+  ALEPH_HIP_BATCH_UNREACHABLE_DEPS = 2
+};
+typedef struct aleph_hip_batch_opts {
+  uint32_t mode;             /* ALEPH_HIP_BATCH_* */
+  uint32_t local_qubits;     /* chunk qubits per pass, 5..13 (0 = 13 for F32, 12 for F64) */
+  uint32_t chunks_per_item;  /* chunks one workgroup takes per claim (0 = 16) */
+  uint32_t groups_per_cu;    /* persistent workgroups per CU (0 = occupancy limit) */
+  uint32_t max_ops_per_pass; /* planner budget (0 = default) */
+  uint32_t spin_limit;       /* bound on a dependency wait, in polls (0 = default) */
+  uint32_t flags;            /* ALEPH_HIP_BATCH_NONTEMPORAL */
+  uint32_t reserved;
+} aleph_hip_batch_opts;
+typedef struct aleph_hip_batch_stats {
+  double plan_ms;
+  double sweep_bytes;        /* algorithmic HBM bytes of one run (2 * 2^n * B per pass) */
+  uint64_t pool_bytes;
+  uint32_t n_instances, n_passes, n_items, n_rounds;
+  uint32_t grid, threads, lds_bytes, mode;
+  uint32_t local_qubits, chunks_per_item, groups_per_cu, reserved;
+} aleph_hip_batch_stats;
+typedef struct aleph_hip_batch_tune {
+  uint32_t mode, local_qubits, chunks_per_item, groups_per_cu;
+  double ms;                 /* best of the timed runs */
+} aleph_hip_batch_tune;
+/* Generic batch: instance i is the program words[word_offsets[i] .. word_offsets[i+1]),
+ * reals[real_offsets[i] .. real_offsets[i+1]) on num_qubits[i] qubits (5..32). */
+aleph_hip_status aleph_hip_batch_create(aleph_hip_ctx* ctx, uint32_t n_instances, const uint32_t* num_qubits,
+                                        uint32_t precision, const uint32_t* words, const uint64_t* word_offsets,
+                                        const double* reals, const uint64_t* real_offsets,
+                                        const aleph_hip_batch_opts* opts, aleph_hip_batch** out);
+/* QAOA MaxCut batch: instance i has edges[2*edge_offsets[i] ..] (pairs u, v) and p layers of
+ * gammas[i*p + l], betas[i*p + l]; program H^n, then per layer CNOT(u,v) Rz(gamma)(v) CNOT(u,v)
+ * on every edge and Rx(beta) on every qubit. */
+aleph_hip_status aleph_hip_batch_create_qaoa(aleph_hip_ctx* ctx, uint32_t n_instances, const uint32_t* num_qubits,
+                                             const uint64_t* edge_offsets, const uint32_t* edges, uint32_t p,
+                                             const double* gammas, const double* betas, uint32_t precision,
+                                             const aleph_hip_batch_opts* opts, aleph_hip_batch** out);
+/* New angles for a QAOA batch (same graphs and p): re-plans and re-uploads. */
+aleph_hip_status aleph_hip_batch_set_qaoa_angles(aleph_hip_ctx* ctx, aleph_hip_batch* b, const double* gammas,
+                                                 const double* betas);
+/* Reset every instance to |0..0> and run every plan (stream-ordered graph replay). */
+aleph_hip_status aleph_hip_batch_run(aleph_hip_ctx* ctx, aleph_hip_batch* b);
+/* Same, synchronised, with the device time of the whole batch in *ms. */
+aleph_hip_status aleph_hip_batch_run_timed(aleph_hip_ctx* ctx, aleph_hip_batch* b, double* ms);
+/* <C>_i = sum_edges (1 - <Z_u Z_v>)/2 = sum_x |psi_x|^2 cut_i(x), accumulated in FP64. */
+aleph_hip_status aleph_hip_batch_expect_maxcut(aleph_hip_ctx* ctx, aleph_hip_batch* b, const uint64_t* edge_offsets,
+                                               const uint32_t* edges, double* out);
+/* Instance i's amplitudes as interleaved complex128 (2 * 2^n doubles). */
+aleph_hip_status aleph_hip_batch_read_state(aleph_hip_ctx* ctx, aleph_hip_batch* b, uint32_t instance,
+                                            double* out);
+aleph_hip_status aleph_hip_batch_stats_get(const aleph_hip_batch* b, aleph_hip_batch_stats* out);
+/* Time candidate configurations (executor, chunk size, chunks per claim) on this batch,
+ * `reps` timed runs each, keep the fastest; results[0..*n_results) lists every candidate. */
+aleph_hip_status aleph_hip_batch_autotune(aleph_hip_ctx* ctx, aleph_hip_batch* b, uint32_t reps,
+                                          aleph_hip_batch_tune* results, uint32_t max_results,
+                                          uint32_t* n_results);
+void aleph_hip_batch_free(aleph_hip_batch* b);
+/* Diagnostics: plan a QAOA batch on the host only (no device; such a batch
+ * cannot run) and view its packed arrays. */
+aleph_hip_status aleph_hip_batch_build_host_qaoa(uint32_t n_instances, const uint32_t* num_qubits,
+                                                 const uint64_t* edge_offsets, const uint32_t* edges, uint32_t p,
+                                                 const double* gammas, const double* betas, uint32_t precision,
+                                                 const aleph_hip_batch_opts* opts, aleph_hip_batch** out,
+                                                 aleph_hip_error* err);
+typedef struct aleph_hip_batch_view {
+  const void* items; uint32_t n_items, item_bytes;          /* BatchItem records, queue order */
+  const void* passes; uint32_t n_passes, pass_bytes;        /* PassDev records */
+  const void* stages; uint32_t n_stages, stage_bytes;
+  const void* ops; uint32_t n_ops, op_bytes;
+  const void* terms; uint32_t n_terms, term_bytes;
+  const uint64_t* offsets; uint32_t n_instances, n_rounds; /* first amplitude of each instance */
+  const uint32_t* round_begin;                              /* n_rounds + 1 item indices */
+  uint32_t threads, lds_bytes;
+  uint64_t total_amps;
+} aleph_hip_batch_view;
+aleph_hip_status aleph_hip_batch_view_get(const aleph_hip_batch* b, aleph_hip_batch_view* out);
+
 #ifdef __cplusplus
 }
 #endif

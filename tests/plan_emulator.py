@@ -54,7 +54,8 @@ def emulate(plan: HostPlan, psi: np.ndarray, precision: int = 64) -> np.ndarray:
                 rg = [int(sd.rg[g]) for g in range(G)]
                 for oi in range(sd.op_begin, sd.op_end):
                     op = plan.ops[oi]
-                    m = np.array(op.m[:32]).view(np.complex128)
+                    raw = np.frombuffer(bytes(op.m), dtype=np.float64 if precision == 64 else np.float32)
+                    m = raw.astype(np.float64)[:32].view(np.complex128)
                     if op.type == 1:
                         psi = _apply_matrix(psi, n, [_bit(rg[op.a])], m[:4].reshape(2, 2), int(op.cmask))
                     elif op.type == 2:

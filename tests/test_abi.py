@@ -42,6 +42,9 @@ def test_struct_layouts():
     assert C.sizeof(L.Gate) == 4 * 4 + 8 + 8 + 24 + 8
     assert C.sizeof(L.PlanOpts) == 8 * 4
     assert C.sizeof(L.Error) == 4 * 5 + 4 + 8 + 48 + 256
+    assert C.sizeof(L.BatchOpts) == 8 * 4
+    assert C.sizeof(L.BatchStats) == 8 * 3 + 12 * 4
+    assert C.sizeof(L.BatchTune) == 4 * 4 + 8
 
 
 def test_no_device_paths_fail_cleanly():

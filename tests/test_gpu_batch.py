@@ -1,0 +1,146 @@
+"""GPU parity of batched execution (aleph_hip_batch_*) through the C ABI.
+
+Every instance of a batch must equal the CPU oracle run of the same circuit
+(FP64 within 1e-12, FP32 within 1e-5), and <C> must equal sum_x p(x) cut(x)
+from the oracle's amplitudes.  Both executors (persistent, rounds) are
+covered, graph replays are checked against the eager oracle result after
+angle updates, and the bounded dependency wait is forced to fail from the
+host with a spin limit of one poll.
+"""
+
+import numpy as np
+import pytest
+
+from aleph_amd import _lib as L
+from aleph_amd import workloads as W
+from aleph_amd.backend import DeviceFailure
+from aleph_amd.batch import HipBatch, maxcut_value
+from tests.helpers import PARITY_TOL, max_abs
+from tests.test_oracle import random_circuit
+
+pytestmark = pytest.mark.gpu
+
+FP32_TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def orc(oracle_lib):
+    return oracle_lib
+
+
+def oracle_amps(orc, circuit):
+    o = orc.OracleSv()
+    st = o.allocate(circuit.num_qubits)
+    o.run(circuit, st)
+    return st.amplitudes()
+
+
+def small_qaoa(count, seed, n_min=6, n_max=16, p=3):
+    return W.qaoa_batch(count, seed, n_min=n_min, n_max=n_max, p=p)
+
+
+def check_qaoa_batch(orc, batch, graphs, gammas, betas, tol):
+    cvals = batch.expect_maxcut(graphs)
+    for i, (n, edges) in enumerate(graphs):
+        want = oracle_amps(orc, W.qaoa_maxcut(n, edges, gammas[i], betas[i]))
+        got = batch.state(i)
+        assert max_abs(got, want) < tol, (i, n)
+        c_want = maxcut_value(n, edges, np.abs(want) ** 2)
+        assert abs(cvals[i] - c_want) < tol * max(1.0, len(edges)) * 10, (i, cvals[i], c_want)
+
+
+@pytest.mark.parametrize("mode", [L.BATCH_PERSISTENT, L.BATCH_ROUNDS])
+@pytest.mark.parametrize("precision", [64, 32])
+def test_qaoa_batch_matches_oracle(hip, orc, mode, precision):
+    graphs, gammas, betas = small_qaoa(14, seed=3 + mode + precision)
+    b = HipBatch.qaoa(hip, graphs, gammas, betas, precision=precision, mode=mode, local_qubits=9,
+                      chunks_per_item=2)
+    b.run()
+    check_qaoa_batch(orc, b, graphs, gammas, betas, PARITY_TOL if precision == 64 else FP32_TOL)
+    b.free()
+
+
+def test_executors_agree_bitwise(hip):
+    graphs, gammas, betas = small_qaoa(20, seed=11, n_min=10, n_max=18)
+    states = []
+    for mode in (L.BATCH_PERSISTENT, L.BATCH_ROUNDS):
+        b = HipBatch.qaoa(hip, graphs, gammas, betas, precision=32, mode=mode, local_qubits=10, chunks_per_item=1)
+        b.run()
+        states.append([b.state(i) for i in range(len(graphs))])
+        b.free()
+    for a, c in zip(*states):
+        assert np.array_equal(a, c)
+
+
+def test_generic_circuit_batch(hip, orc):
+    rng = np.random.default_rng(5)
+    circuits = [random_circuit(int(n), 40, rng, kq=False) for n in rng.integers(5, 12, size=9)]
+    circuits.append(W.brickwall(13, 6, seed=2))
+    circuits.append(W.hea(11, 3, seed=9))
+    b = HipBatch.circuits(hip, circuits, precision=64, local_qubits=8)
+    b.run()
+    for i, c in enumerate(circuits):
+        assert max_abs(b.state(i), oracle_amps(orc, c)) < PARITY_TOL, i
+    b.free()
+
+
+def test_graph_replay_with_angle_updates(hip, orc):
+    """The captured graph is replayed after every angle update, with eager
+    device work and synchronisations in between; each replay is checked."""
+    graphs, gammas, betas = small_qaoa(8, seed=21, n_min=8, n_max=14, p=2)
+    b = HipBatch.qaoa(hip, graphs, gammas, betas, precision=64, local_qubits=8)
+    rng = np.random.default_rng(0)
+    for rep in range(3):
+        if rep:
+            gammas = rng.uniform(0, np.pi, size=gammas.shape)
+            betas = rng.uniform(0, np.pi, size=betas.shape)
+            b.set_qaoa_angles(gammas, betas)
+        b.run()
+        st = hip.allocate(6)          # unrelated eager work between replays
+        hip.norm_sqr(st)
+        st.free()
+        hip.synchronize()
+        check_qaoa_batch(orc, b, graphs, gammas, betas, PARITY_TOL)
+        b.run()                       # a second replay of the same graph
+        check_qaoa_batch(orc, b, graphs, gammas, betas, PARITY_TOL)
+    b.free()
+
+
+def test_bounded_wait_failure_is_reported(hip, orc):
+    """With every dependency made unsatisfiable from the host and a spin
+    limit of 16 polls, the waits give up: every workgroup still exits, the
+    run reports a device failure, and the device keeps working."""
+    graphs, gammas, betas = W.qaoa_batch(2, seed=5, n_min=16, n_max=16, p=2)
+    b = HipBatch.qaoa(hip, graphs, gammas, betas, precision=32, mode=L.BATCH_PERSISTENT, local_qubits=10,
+                      chunks_per_item=1, spin_limit=16, flags=L.BATCH_UNREACHABLE_DEPS)
+    with pytest.raises(DeviceFailure):
+        b.run()
+    b.free()
+    good = HipBatch.qaoa(hip, graphs, gammas, betas, precision=32, mode=L.BATCH_PERSISTENT, local_qubits=10)
+    good.run()
+    check_qaoa_batch(orc, good, graphs, gammas, betas, FP32_TOL)
+    good.free()
+
+
+def test_batch_stats_and_autotune(hip, orc):
+    graphs, gammas, betas = small_qaoa(10, seed=31, n_min=10, n_max=16, p=2)
+    b = HipBatch.qaoa(hip, graphs, gammas, betas, precision=32)
+    st = b.stats()
+    assert st["n_instances"] == 10 and st["n_passes"] >= 10 and st["n_items"] >= st["n_passes"]
+    assert st["sweep_bytes"] > 0 and st["pool_bytes"] == 8 * sum(1 << n for n, _ in graphs)
+    res = b.autotune(reps=1)
+    assert len(res) == 18 and all(r["ms"] > 0 for r in res)
+    best = min(res, key=lambda r: r["ms"])
+    st = b.stats()
+    assert (st["mode"], st["local_qubits"], st["chunks_per_item"]) == (best["mode"], best["local_qubits"],
+                                                                       best["chunks_per_item"])
+    b.run()
+    check_qaoa_batch(orc, b, graphs, gammas, betas, FP32_TOL)
+    b.free()
+
+
+def test_batch_rejects_bad_input(hip):
+    with pytest.raises(Exception):
+        HipBatch.qaoa(hip, [(4, [(0, 1)])], [[0.1]], [[0.2]])          # n < 5
+    with pytest.raises(Exception):
+        HipBatch.qaoa(hip, [(6, [(0, 9)])], [[0.1]], [[0.2]])          # edge out of range

@@ -153,3 +153,19 @@ def test_non_identity_cnot_runs_are_kept():
     c = W.hea(10, 3, seed=1)
     a, b = HostPlan(c), HostPlan(c, fuse_diag=False)
     assert [op.type for op in a.ops] == [op.type for op in b.ops]
+
+
+@pytest.mark.parametrize("name", ["brickwall", "qaoa", "random"])
+def test_emulated_fp32_plans(oracle_lib, name):
+    """FP32 plans carry their op matrices as floats; emulating them in
+    double precision from those floats stays within the FP32 budget."""
+    n = 9
+    if name == "brickwall":
+        c = W.brickwall(n, 8, seed=4)
+    elif name == "qaoa":
+        c = W.qaoa_maxcut(8, W.random_3_regular(8, 2), [0.5, 1.7], [0.3, 2.2])
+    else:
+        c = random_circuit(n, 50, np.random.default_rng(8))
+    psi = W.gaussian_state(c.num_qubits, 1)
+    plan = HostPlan(c, precision=32)
+    assert max_abs(emulate(plan, psi, precision=32), oracle_state(oracle_lib, c, psi)) < 1e-5
