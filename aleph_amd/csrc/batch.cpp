@@ -112,7 +112,7 @@ void Batch::drop_graph() {
 void Batch::plan() {
   const auto t0 = std::chrono::steady_clock::now();
   const uint32_t N = size();
-  const uint32_t m_cfg = cfg.local_qubits ? cfg.local_qubits : (precision == ALEPH_HIP_F32 ? 13u : 12u);
+  const uint32_t m_cfg = cfg.local_qubits ? cfg.local_qubits : 12u;
   if (m_cfg < kBatchRegBits + 1 || m_cfg > (uint32_t)kMaxLocalBits) {
     throw make_failure(ALEPH_HIP_ERR_INVALID_ARGUMENT, "batch", "local_qubits must be in [5, 13]");
   }

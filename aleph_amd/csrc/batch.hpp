@@ -19,7 +19,7 @@ enum BatchMode : uint32_t { BATCH_PERSISTENT = 0, BATCH_ROUNDS = 1 };
 
 struct BatchConfig {
   uint32_t mode = BATCH_PERSISTENT;
-  uint32_t local_qubits = 0;     // chunk qubits per pass (0: 13 for F32, 12 for F64)
+  uint32_t local_qubits = 0;     // chunk qubits per pass (0: 12)
   uint32_t chunks_per_item = 0;  // 0: 16
   uint32_t groups_per_cu = 0;    // persistent workgroups per CU (0: occupancy limit)
   uint32_t max_ops = 0;          // pass budget (0: planner default)

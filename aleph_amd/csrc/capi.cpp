@@ -1035,9 +1035,7 @@ aleph_hip_status aleph_hip_batch_autotune(aleph_hip_ctx* c, aleph_hip_batch* h, 
     Batch& b = *h->b;
     reps = std::max(1u, reps);
     const BatchConfig base = b.cfg;
-    const bool f32 = b.precision == ALEPH_HIP_F32;
-    const uint32_t ms_default = f32 ? 13u : 12u;
-    std::vector<uint32_t> chunk_bits = {ms_default - 2, ms_default - 1, ms_default};
+    std::vector<uint32_t> chunk_bits = {10u, 11u, 12u};
     const uint32_t cpis[] = {4, 16, 64};
     const uint32_t modes[] = {ALEPH_HIP_BATCH_PERSISTENT, ALEPH_HIP_BATCH_ROUNDS};
     std::vector<aleph_hip_batch_tune> all;

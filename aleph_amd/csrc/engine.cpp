@@ -823,7 +823,10 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
 void plan_rule(Plan& p, std::vector<LGate> lg, uint32_t n, const PlanOptions& o) {
   PlannerCtx pc;
   pc.n = n;
-  const uint32_t mdef = p.precision == ALEPH_HIP_F64 ? 12u : 13u;
+  // 2^12-amplitude chunks (256-thread workgroups) for both precisions: FP32
+  // chunks of 2^13 need 512-thread workgroups, of which one fits per CU,
+  // and ran brickwall passes 1.8x slower on MI355X.
+  const uint32_t mdef = 12u;
   pc.m = std::min<uint32_t>(o.local_qubits ? o.local_qubits : mdef, n);
   pc.m = std::min<uint32_t>(pc.m, kMaxLocalBits);
   pc.G = o.reg_bits ? o.reg_bits : (uint32_t)kRegBits;

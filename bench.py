@@ -38,6 +38,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the FP32 and QFT legs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline time budget")
+    ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 traffic measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched QAOA leg")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
@@ -99,8 +103,118 @@ def cpu_baseline(circuit, budget_s: float):
     }
 
 
+FUSED_KERNEL = "k_fused_pass<double"
+
+
+def traffic_child(args) -> None:
+    """Run under rocprofv3 --pmc: one |0..0> reset (k_init_basis writes the
+    whole state), one execution of the plan, one norm (k_reduce_branch reads
+    the whole state).  The two full-state kernels calibrate the counters."""
+    from aleph_amd import workloads as W
+    from aleph_amd.backend import HipSvBackend
+    b = HipSvBackend(seed=1)
+    c = W.brickwall(args.n, args.depth, seed=args.seed)
+    plan = b.compile(c, local_qubits=args.local_qubits, max_ops=args.max_ops)
+    st = b.allocate(args.n)
+    plan.execute(st)
+    b.norm_sqr(st)
+    b.synchronize()
+
+
+def _pmc_pass(counter: str, args, out_dir: str) -> dict:
+    """Per-kernel mean of one counter (KiB per dispatch) from a rocprofv3 pass."""
+    import csv
+    import subprocess
+    cmd = ["timeout", "-s", "KILL", "150", "rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", out_dir,
+           "-o", counter, "--", sys.executable, os.path.abspath(__file__), "--traffic-child", "--n", str(args.n),
+           "--depth", str(args.depth), "--seed", str(args.seed), "--local-qubits", str(args.local_qubits),
+           "--max-ops", str(args.max_ops)]
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd="/tmp")
+    sums, counts = {}, {}
+    for root, _, files in os.walk(out_dir):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                for row in csv.DictReader(open(os.path.join(root, f))):
+                    if row["Counter_Name"] != counter:
+                        continue
+                    name = row["Kernel_Name"]
+                    key = (name, row["Dispatch_Id"])
+                    sums[key] = sums.get(key, 0.0) + float(row["Counter_Value"])
+    per_kernel = {}
+    for (name, _), v in sums.items():
+        per_kernel.setdefault(name, []).append(v)
+    return {k: float(np.mean(v)) for k, v in per_kernel.items()}
+
+
+def measure_traffic(args, state_bytes: float):
+    """HBM bytes per fused-pass launch from two rocprofv3 passes (FETCH_SIZE,
+    WRITE_SIZE), each scaled so that the full-state kernels read / write
+    exactly the state: the write factor from k_init_basis, the fetch factor
+    from k_reduce_branch."""
+    import shutil
+    import tempfile
+    if shutil.which("rocprofv3") is None:
+        return None
+    tmp = tempfile.mkdtemp(prefix="aleph_pmc_")
+    try:
+        fetch = _pmc_pass("FETCH_SIZE", args, os.path.join(tmp, "fetch"))
+        write = _pmc_pass("WRITE_SIZE", args, os.path.join(tmp, "write"))
+    except Exception as e:  # profiler unavailable or refused: no traffic figure
+        log(f"bench: traffic measurement skipped ({e})")
+        return None
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+    def pick(d, key):
+        vals = [v for k, v in d.items() if key in k]
+        return float(np.mean(vals)) if vals else None
+    f_fused, w_fused = pick(fetch, FUSED_KERNEL), pick(write, FUSED_KERNEL)
+    f_cal, w_cal = pick(fetch, "k_reduce_branch<double"), pick(write, "k_init_basis<double")
+    if None in (f_fused, w_fused, f_cal, w_cal) or f_cal <= 0 or w_cal <= 0:
+        return None
+    kib = state_bytes / 1024.0
+    f_scale, w_scale = kib / f_cal, kib / w_cal
+    return {"bytes": (f_fused * f_scale + w_fused * w_scale) * 1024.0,
+            "fetch_bytes": f_fused * f_scale * 1024.0, "write_bytes": w_fused * w_scale * 1024.0,
+            "fetch_scale": f_scale, "write_scale": w_scale}
+
+
+def batch_leg(args) -> dict:
+    """Batched QAOA MaxCut: seeded instances, n even in [12, 24], p = 4,
+    FP32 amplitudes, <C> in FP64; tile shape and executor autotuned on every
+    8th instance, then the whole batch timed."""
+    from aleph_amd import workloads as W
+    from aleph_amd.backend import HipSvBackend
+    from aleph_amd.batch import HipBatch
+    graphs, ga, be = W.qaoa_batch(args.batch_count, seed=args.seed)
+    hip = HipSvBackend(seed=1, precision=32)
+    sub = list(range(0, len(graphs), 8))
+    tuner = HipBatch.qaoa(hip, [graphs[i] for i in sub], ga[sub], be[sub], precision=32)
+    cands = tuner.autotune(reps=2)
+    tuner.free()
+    best = min(cands, key=lambda r: r["ms"])
+    b = HipBatch.qaoa(hip, graphs, ga, be, precision=32, mode=best["mode"], local_qubits=best["local_qubits"],
+                      chunks_per_item=best["chunks_per_item"])
+    b.run()
+    times = [b.run_timed() for _ in range(3)]
+    ms = min(times)
+    cvals = b.expect_maxcut()
+    st = b.stats()
+    b.free()
+    return {"instances": len(graphs), "ms": ms, "instances_per_s": len(graphs) / (ms * 1e-3),
+            "hbm_gbs": st["sweep_bytes"] / (ms * 1e-3) / 1e9, "sweep_bytes": st["sweep_bytes"],
+            "passes": st["n_passes"], "items": st["n_items"], "pool_gb": st["pool_bytes"] / 1e9,
+            "plan_ms": st["plan_ms"], "mean_cut": float(np.mean(cvals)),
+            "tuned": {"mode": ["persistent", "rounds"][best["mode"]], "local_qubits": best["local_qubits"],
+                      "chunks_per_item": best["chunks_per_item"]},
+            "config": "QAOA MaxCut, seeded random 3-regular graphs, n even in [12,24], p=4, FP32, <C> FP64"}
+
+
 def main():
     args = parse()
+    if args.traffic_child:
+        traffic_child(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this backend runs on one GPU; --gpus must be 1 (sharding is a design note)"}))
         sys.exit(2)
@@ -161,6 +275,16 @@ def main():
         log(f"bench: qft fp64 {msq:.2f} ms/circuit passes={stq['n_passes']}")
         out["qft_fp64"] = {"ms_per_step": msq, "passes": stq["n_passes"], "gates": len(q.gates()),
                            "hbm_gbs": stq["sweep_bytes"] / (msq / 1e3) / 1e9}
+    if not args.no_batch:
+        log("bench: batched qaoa")
+        out["qaoa_batch"] = batch_leg(args)
+        log(f"bench: qaoa batch {out['qaoa_batch']['ms']:.1f} ms")
+    if not args.no_pmc:
+        log("bench: rocprofv3 traffic passes")
+        tr = measure_traffic(args, float(1 << n) * 16.0)
+        if tr is not None:
+            out["roofline"]["traffic"] = tr["bytes"]
+            out["roofline"]["traffic_detail"] = tr
     if not args.no_cpu:
         log("bench: cpu baseline")
         cpu = cpu_baseline(circuit, args.cpu_seconds)

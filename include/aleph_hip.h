@@ -173,7 +173,7 @@ enum { ALEPH_HIP_PLANNER_UNFUSED = 0, ALEPH_HIP_PLANNER_RULE = 1, ALEPH_HIP_PLAN
 
 typedef struct aleph_hip_plan_opts {
   uint32_t planner;         /* ALEPH_HIP_PLANNER_* */
-  uint32_t local_qubits;    /* qubits per LDS chunk (0 = default for the precision) */
+  uint32_t local_qubits;    /* qubits per LDS chunk (0 = 12) */
   uint32_t max_ops_per_pass;/* compute budget per pass, in 1q-gate equivalents (0 = default) */
   uint32_t coalesce_bits;   /* low qubits kept in every chunk for coalescing (0 = default) */
   uint32_t fuse_1q;         /* merge adjacent 1q gates on one qubit before planning */
@@ -270,7 +270,7 @@ enum {
 };
 typedef struct aleph_hip_batch_opts {
   uint32_t mode;             /* ALEPH_HIP_BATCH_* */
-  uint32_t local_qubits;     /* chunk qubits per pass, 5..13 (0 = 13 for F32, 12 for F64) */
+  uint32_t local_qubits;     /* chunk qubits per pass, 5..13 (0 = 12) */
   uint32_t chunks_per_item;  /* chunks one workgroup takes per claim (0 = 16) */
   uint32_t groups_per_cu;    /* persistent workgroups per CU (0 = occupancy limit) */
   uint32_t max_ops_per_pass; /* planner budget (0 = default) */
