@@ -62,7 +62,13 @@ class PlanOpts(C.Structure):
 
 class PlanStats(C.Structure):
     _fields_ = [("plan_ms", f64), ("input_gates", u32), ("n_passes", u32), ("n_stages", u32), ("n_ops", u32),
-                ("n_launches", u32), ("reserved", u32), ("sweep_bytes", f64)]
+                ("n_launches", u32), ("planner_used", u32), ("sweep_bytes", f64), ("chosen_local_qubits", u32),
+                ("chosen_max_ops", u32), ("chosen_flags", u32), ("n_candidates", u32), ("predicted_us", f64)]
+
+
+MODEL_INPUTS = 16
+MODEL_PARAMS = 9473
+PF32 = C.POINTER(C.c_float)
 
 
 BATCH_PERSISTENT, BATCH_ROUNDS = 0, 1
@@ -154,6 +160,15 @@ SIGNATURES = {
     "aleph_hip_batch_stats_get": (i32, [P, C.POINTER(BatchStats)]),
     "aleph_hip_batch_autotune": (i32, [P, P, u32, C.POINTER(BatchTune), u32, PU32]),
     "aleph_hip_batch_free": (None, [P]),
+    "aleph_hip_model_create": (i32, [P, PF32, u64, C.POINTER(P)]),
+    "aleph_hip_model_get_params": (i32, [P, P, PF32]),
+    "aleph_hip_model_set_params": (i32, [P, P, PF32]),
+    "aleph_hip_model_predict": (i32, [P, P, PF32, u32, PF32]),
+    "aleph_hip_model_gradient": (i32, [P, P, PF32, PF32, u32, PF32, PF32]),
+    "aleph_hip_model_train": (i32, [P, P, PF32, PF32, u32, u32, u32, C.c_float, u64, PF32]),
+    "aleph_hip_model_free": (None, [P]),
+    "aleph_hip_ctx_set_cost_model": (i32, [P, P]),
+    "aleph_hip_plan_features": (i32, [P, PF32, PU32, u32, PU32]),
     "aleph_hip_batch_build_host_qaoa": (i32, [u32, PU32, PU64, PU32, u32, PF64, PF64, u32, C.POINTER(BatchOpts),
                                               C.POINTER(P), C.POINTER(Error)]),
     "aleph_hip_batch_view_get": (i32, [P, C.POINTER(BatchView)]),

@@ -151,6 +151,10 @@ class HipPlan:
     def __init__(self, backend: "HipSvBackend", handle, num_qubits: int, precision: int):
         self._b, self._h, self.num_qubits, self.precision = backend, handle, num_qubits, precision
 
+    @property
+    def handle(self):
+        return self._h
+
     def execute(self, state: HipState) -> None:
         self._b._check(L.load().aleph_hip_plan_execute(self._b._ctx, self._h, state.handle))
 

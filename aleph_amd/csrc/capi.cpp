@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -30,6 +31,7 @@ struct aleph_hip_ctx {
   double* d_scratch = nullptr;      // reductions + small matrices
   size_t scratch_bytes = 0;
   double* h_pinned = nullptr;       // 64 doubles of pinned readback
+  float* d_model = nullptr;         // cost model of the learned planner (kMlpParams floats) or null
 };
 
 struct aleph_hip_state {
@@ -45,6 +47,13 @@ struct aleph_hip_plan {
 
 struct aleph_hip_batch {
   std::unique_ptr<Batch> b;
+};
+
+struct aleph_hip_model {
+  float* d = nullptr;   // kMlpParams floats on the device
+  ~aleph_hip_model() {
+    if (d) (void)hipFree(d);
+  }
 };
 
 namespace {
@@ -283,6 +292,7 @@ void aleph_hip_ctx_destroy(aleph_hip_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->d_scratch) (void)hipFree(c->d_scratch);
+  if (c->d_model) (void)hipFree(c->d_model);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -564,8 +574,72 @@ aleph_hip_status aleph_hip_apply_swap(aleph_hip_ctx* c, aleph_hip_state* st, uin
 }
 
 namespace {
-std::unique_ptr<aleph_hip_plan> make_plan(uint32_t n, uint32_t precision, const uint32_t* words, size_t n_words,
-                                          const double* reals, size_t n_reals, const aleph_hip_plan_opts* opts) {
+// Candidate search of the learned planner: every combination of chunk
+// qubits, pass budget and 4x4 fusion is planned by the rule planner, the
+// cost model predicts each fused launch (ln microseconds), and the plan
+// with the smallest predicted total is kept.
+std::unique_ptr<Plan> plan_learned(aleph_hip_ctx* c, const std::vector<ProgInstr>& prog, uint32_t n,
+                                   uint32_t precision, PlanOptions base) {
+  const auto t0 = std::chrono::steady_clock::now();
+  struct Cand {
+    uint32_t m, budget;
+    bool f2q;
+    std::unique_ptr<Plan> p;
+    double us = 0.0;
+  };
+  std::vector<Cand> cands;
+  for (uint32_t m : {10u, 11u, 12u}) {
+    if (m > n && m != 10u) continue;
+    for (uint32_t budget : {12u, 24u, 48u}) {
+      for (bool f2q : {true, false}) {
+        PlanOptions po = base;
+        po.planner = ALEPH_HIP_PLANNER_RULE;
+        po.local_qubits = std::min(m, n);
+        po.max_ops = budget;
+        po.fuse_2q = f2q;
+        cands.push_back(Cand{m, budget, f2q, build_plan(prog, n, precision, po)});
+      }
+    }
+  }
+  std::vector<float> rows;
+  std::vector<uint32_t> owner;
+  for (uint32_t k = 0; k < cands.size(); k++) {
+    std::vector<uint32_t> launch;
+    const size_t before = rows.size() / kPlanFeatures;
+    plan_features(*cands[k].p, rows, launch);
+    owner.insert(owner.end(), rows.size() / kPlanFeatures - before, k);
+    cands[k].us = plan_standalone_us(*cands[k].p);
+  }
+  const uint32_t R = (uint32_t)owner.size();
+  if (R) {
+    DevBuf dx(rows.size() * sizeof(float));
+    DevBuf dy(R * sizeof(float));
+    ALEPH_HIP_CHECK(hipMemcpyAsync(dx.p, rows.data(), rows.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    launch_mlp_forward(c->stream, c->d_model, (const float*)dx.p, R, (float*)dy.p);
+    std::vector<float> pred(R);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(pred.data(), dy.p, R * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (uint32_t r = 0; r < R; r++) cands[owner[r]].us += std::exp((double)std::min(pred[r], 40.f));
+  }
+  size_t best = 0;
+  for (size_t k = 1; k < cands.size(); k++)
+    if (cands[k].us < cands[best].us) best = k;
+  std::unique_ptr<Plan> p = std::move(cands[best].p);
+  p->opts.planner = ALEPH_HIP_PLANNER_LEARNED;
+  aleph_hip_plan_stats& st = p->stats;
+  st.planner_used = ALEPH_HIP_PLANNER_LEARNED;
+  st.chosen_local_qubits = std::min(cands[best].m, n);
+  st.chosen_max_ops = cands[best].budget;
+  st.chosen_flags = cands[best].f2q ? 0u : (uint32_t)ALEPH_HIP_PLAN_NO_FUSE_2Q;
+  st.n_candidates = (uint32_t)cands.size();
+  st.predicted_us = cands[best].us;
+  st.plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return p;
+}
+
+std::unique_ptr<aleph_hip_plan> make_plan(aleph_hip_ctx* c, uint32_t n, uint32_t precision, const uint32_t* words,
+                                          size_t n_words, const double* reals, size_t n_reals,
+                                          const aleph_hip_plan_opts* opts) {
     need(precision == ALEPH_HIP_F64 || precision == ALEPH_HIP_F32, "precision must be 64 or 32");
     need(n >= 1 && n <= 40, "qubit count must be in [1, 40]");
     need(n_words == 0 || words != nullptr, "null words");
@@ -589,7 +663,14 @@ std::unique_ptr<aleph_hip_plan> make_plan(uint32_t n, uint32_t precision, const 
          "local_qubits must be in [reg_bits, 13]");
     const std::vector<ProgInstr> prog = decode_program(words, n_words, reals, n_reals, n);
     auto h = std::make_unique<aleph_hip_plan>();
-    h->plan = build_plan(prog, n, precision, po);
+    if (po.planner == ALEPH_HIP_PLANNER_LEARNED && c != nullptr && c->d_model != nullptr &&
+        n >= (po.reg_bits ? po.reg_bits : (uint32_t)kRegBits)) {
+      h->plan = plan_learned(c, prog, n, precision, po);
+    } else {
+      h->plan = build_plan(prog, n, precision, po);
+      h->plan->stats.planner_used =
+          po.planner == ALEPH_HIP_PLANNER_UNFUSED ? ALEPH_HIP_PLANNER_UNFUSED : ALEPH_HIP_PLANNER_RULE;
+    }
     return h;
 }
 }  // namespace
@@ -600,7 +681,7 @@ aleph_hip_status aleph_hip_plan_create(aleph_hip_ctx* c, uint32_t n, uint32_t pr
   return guarded(c, [&] {
     need(c != nullptr && out != nullptr, "null argument");
     *out = nullptr;
-    auto h = make_plan(n, precision, words, n_words, reals, n_reals, opts);
+    auto h = make_plan(c, n, precision, words, n_words, reals, n_reals, opts);
     h->plan->upload(c->stream);
     *out = h.release();
   });
@@ -612,7 +693,7 @@ aleph_hip_status aleph_hip_plan_build_host(uint32_t n, uint32_t precision, const
   if (!out) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
   *out = nullptr;
   try {
-    *out = make_plan(n, precision, words, n_words, reals, n_reals, opts).release();
+    *out = make_plan(nullptr, n, precision, words, n_words, reals, n_reals, opts).release();
     return ALEPH_HIP_OK;
   } catch (const Failure& f) {
     if (err) {
@@ -1141,6 +1222,176 @@ aleph_hip_status aleph_hip_batch_view_get(const aleph_hip_batch* h, aleph_hip_ba
   v.lds_bytes = (uint32_t)b.launch.lds_bytes;
   v.total_amps = b.total_amps;
   *out = v;
+  return ALEPH_HIP_OK;
+}
+
+// ---- cost model ----
+
+aleph_hip_status aleph_hip_model_create(aleph_hip_ctx* c, const float* params, uint64_t seed, aleph_hip_model** out) {
+  return guarded(c, [&] {
+    need(out != nullptr, "null output");
+    std::vector<float> p(kMlpParams, 0.f);
+    if (params) {
+      std::copy(params, params + kMlpParams, p.begin());
+    } else {
+      // He initialisation from SplitMix64 + Box-Muller; biases zero.
+      uint64_t s = seed;
+      auto u01 = [&] {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+      };
+      auto normal = [&] { return std::sqrt(-2.0 * std::log(u01())) * std::cos(6.283185307179586 * u01()); };
+      const int H = kMlpHidden, I = kMlpIn;
+      size_t o = 0;
+      auto fill = [&](int fan_in, int count) {
+        const double sd = std::sqrt(2.0 / fan_in);
+        for (int k = 0; k < count; k++) p[o++] = (float)(sd * normal());
+      };
+      fill(I, I * H);
+      o += H;
+      fill(H, H * H);
+      o += H;
+      fill(H, H * H);
+      o += H;
+      fill(H, H);
+      o += 1;
+    }
+    auto m = std::make_unique<aleph_hip_model>();
+    ALEPH_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&m->d), kMlpParams * sizeof(float)));
+    ALEPH_HIP_CHECK(hipMemcpy(m->d, p.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+    *out = m.release();
+  });
+}
+
+aleph_hip_status aleph_hip_model_get_params(aleph_hip_ctx* c, const aleph_hip_model* m, float* out) {
+  return guarded(c, [&] {
+    need(m != nullptr && out != nullptr, "null argument");
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    ALEPH_HIP_CHECK(hipMemcpy(out, m->d, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+aleph_hip_status aleph_hip_model_set_params(aleph_hip_ctx* c, aleph_hip_model* m, const float* params) {
+  return guarded(c, [&] {
+    need(m != nullptr && params != nullptr, "null argument");
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    ALEPH_HIP_CHECK(hipMemcpy(m->d, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+  });
+}
+
+aleph_hip_status aleph_hip_model_predict(aleph_hip_ctx* c, const aleph_hip_model* m, const float* X, uint32_t rows,
+                                         float* out) {
+  return guarded(c, [&] {
+    need(m != nullptr && (rows == 0 || (X != nullptr && out != nullptr)), "null argument");
+    if (rows == 0) return;
+    DevBuf dx((size_t)rows * kMlpIn * sizeof(float));
+    DevBuf dy((size_t)rows * sizeof(float));
+    ALEPH_HIP_CHECK(hipMemcpyAsync(dx.p, X, (size_t)rows * kMlpIn * sizeof(float), hipMemcpyHostToDevice,
+                                   c->stream));
+    launch_mlp_forward(c->stream, m->d, (const float*)dx.p, rows, (float*)dy.p);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(out, dy.p, rows * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+aleph_hip_status aleph_hip_model_gradient(aleph_hip_ctx* c, const aleph_hip_model* m, const float* X, const float* y,
+                                          uint32_t rows, float* grads, float* loss) {
+  return guarded(c, [&] {
+    need(m != nullptr && grads != nullptr && loss != nullptr && (rows == 0 || (X && y)), "null argument");
+    DevBuf dx(std::max<size_t>(rows, 1) * kMlpIn * sizeof(float));
+    DevBuf dy(std::max<size_t>(rows, 1) * sizeof(float));
+    DevBuf dg(kMlpParams * sizeof(float));
+    DevBuf dl(sizeof(float));
+    if (rows) {
+      ALEPH_HIP_CHECK(hipMemcpyAsync(dx.p, X, (size_t)rows * kMlpIn * sizeof(float), hipMemcpyHostToDevice,
+                                     c->stream));
+      ALEPH_HIP_CHECK(hipMemcpyAsync(dy.p, y, rows * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    }
+    launch_mlp_grad(c->stream, m->d, (const float*)dx.p, (const float*)dy.p, rows, (float*)dg.p, (float*)dl.p);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(grads, dg.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    ALEPH_HIP_CHECK(hipMemcpyAsync(loss, dl.p, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+aleph_hip_status aleph_hip_model_train(aleph_hip_ctx* c, aleph_hip_model* m, const float* X, const float* y,
+                                       uint32_t rows, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
+                                       float* loss) {
+  return guarded(c, [&] {
+    need(m != nullptr && loss != nullptr && rows > 0 && X != nullptr && y != nullptr, "null argument");
+    need(batch > 0 && std::isfinite(lr) && lr > 0.f, "batch and learning rate must be positive");
+    const size_t xb = (size_t)rows * kMlpIn * sizeof(float);
+    DevBuf dx(xb), dy(rows * sizeof(float)), dg(kMlpParams * sizeof(float)), dl(sizeof(float));
+    DevBuf dm(kMlpParams * sizeof(float)), dv(kMlpParams * sizeof(float));
+    std::vector<float> zeros(kMlpParams, 0.f);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(dm.p, zeros.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    ALEPH_HIP_CHECK(hipMemcpyAsync(dv.p, zeros.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    std::vector<uint32_t> perm(rows);
+    for (uint32_t i = 0; i < rows; i++) perm[i] = i;
+    std::vector<float> xs((size_t)rows * kMlpIn), ys(rows);
+    uint64_t s = seed;
+    auto next = [&] {
+      s += 0x9E3779B97F4A7C15ull;
+      uint64_t z = s;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      return z ^ (z >> 31);
+    };
+    uint32_t t = 0;
+    for (uint32_t e = 0; e < epochs; e++) {
+      for (uint32_t i = rows - 1; i > 0; i--) std::swap(perm[i], perm[next() % (i + 1)]);
+      for (uint32_t i = 0; i < rows; i++) {
+        std::copy(X + (size_t)perm[i] * kMlpIn, X + (size_t)(perm[i] + 1) * kMlpIn, xs.begin() + (size_t)i * kMlpIn);
+        ys[i] = y[perm[i]];
+      }
+      ALEPH_HIP_CHECK(hipMemcpyAsync(dx.p, xs.data(), xb, hipMemcpyHostToDevice, c->stream));
+      ALEPH_HIP_CHECK(hipMemcpyAsync(dy.p, ys.data(), rows * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      for (uint32_t b0 = 0; b0 < rows; b0 += batch) {
+        const uint32_t bs = std::min(batch, rows - b0);
+        launch_mlp_grad(c->stream, m->d, (const float*)dx.p + (size_t)b0 * kMlpIn, (const float*)dy.p + b0, bs,
+                        (float*)dg.p, (float*)dl.p);
+        launch_mlp_adam(c->stream, m->d, (const float*)dg.p, (float*)dm.p, (float*)dv.p, lr, 0.9f, 0.999f, 1e-8f,
+                        ++t);
+      }
+      ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));  // host buffers are rewritten next epoch
+    }
+    ALEPH_HIP_CHECK(hipMemcpyAsync(dx.p, X, xb, hipMemcpyHostToDevice, c->stream));
+    ALEPH_HIP_CHECK(hipMemcpyAsync(dy.p, y, rows * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    launch_mlp_grad(c->stream, m->d, (const float*)dx.p, (const float*)dy.p, rows, (float*)dg.p, (float*)dl.p);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(loss, dl.p, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+void aleph_hip_model_free(aleph_hip_model* m) { delete m; }
+
+aleph_hip_status aleph_hip_ctx_set_cost_model(aleph_hip_ctx* c, const aleph_hip_model* m) {
+  return guarded(c, [&] {
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (!m) {
+      if (c->d_model) ALEPH_HIP_CHECK(hipFree(c->d_model));
+      c->d_model = nullptr;
+      return;
+    }
+    if (!c->d_model) ALEPH_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&c->d_model), kMlpParams * sizeof(float)));
+    ALEPH_HIP_CHECK(hipMemcpy(c->d_model, m->d, kMlpParams * sizeof(float), hipMemcpyDeviceToDevice));
+  });
+}
+
+aleph_hip_status aleph_hip_plan_features(const aleph_hip_plan* plan, float* out, uint32_t* launch, uint32_t max_rows,
+                                         uint32_t* rows) {
+  if (!plan || !plan->plan || !rows) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  std::vector<float> f;
+  std::vector<uint32_t> l;
+  plan_features(*plan->plan, f, l);
+  *rows = (uint32_t)l.size();
+  const uint32_t n = std::min<uint32_t>(max_rows, (uint32_t)l.size());
+  if (out) std::copy(f.begin(), f.begin() + (size_t)n * kPlanFeatures, out);
+  if (launch) std::copy(l.begin(), l.begin() + n, launch);
   return ALEPH_HIP_OK;
 }
 

@@ -1029,6 +1029,60 @@ std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog, uint32_t n,
   return p;
 }
 
+void plan_features(const Plan& p, std::vector<float>& rows, std::vector<uint32_t>& launch) {
+  for (size_t i = 0; i < p.steps.size(); i++) {
+    const Step& s = p.steps[i];
+    if (s.kind != Step::FUSED) continue;
+    const PassDev& pd = p.passes[s.pass];
+    float n1q = 0, n2q = 0, ndiag = 0, nsign = 0, nphase = 0, nreg = 0, ngen = 0, nctl = 0;
+    for (uint32_t st = pd.stage_begin; st < pd.stage_begin + pd.n_stages; st++) {
+      for (uint32_t o = p.stages[st].op_begin; o < p.stages[st].op_end; o++) {
+        const DevOp& op = p.ops[o];
+        switch (op.type) {
+          case OP_1Q: n1q += 1; break;
+          case OP_2Q: n2q += 1; break;
+          case OP_DIAG: ndiag += 1; break;
+          case OP_SIGN: nsign += 1; break;
+          case OP_PHASE:
+            nphase += 1;
+            nreg += (float)op.n_terms;
+            ngen += (float)op.b;
+            break;
+          default: break;
+        }
+        if (op.type != OP_SIGN && op.type != OP_PHASE && op.cmask != 0) nctl += 1;
+      }
+    }
+    const float f[kPlanFeatures] = {(float)p.n / 32.f,
+                                    p.precision == ALEPH_HIP_F64 ? 1.f : 0.f,
+                                    (float)pd.m / 13.f,
+                                    (float)pd.n_stages / 8.f,
+                                    n1q / 32.f,
+                                    n2q / 32.f,
+                                    ndiag / 16.f,
+                                    nsign / 32.f,
+                                    nphase / 8.f,
+                                    nreg / 128.f,
+                                    ngen / 64.f,
+                                    (pd.features & kFeat2Q) ? 1.f : 0.f,
+                                    (pd.features & kFeatPhase) ? 1.f : 0.f,
+                                    (pd.features & kFeatNontemporal) ? 1.f : 0.f,
+                                    (float)pd.reg_bits - 4.f,
+                                    nctl / 32.f};
+    rows.insert(rows.end(), f, f + kPlanFeatures);
+    launch.push_back((uint32_t)i);
+  }
+}
+
+double plan_standalone_us(const Plan& p) {
+  const double bpa = p.precision == ALEPH_HIP_F64 ? 16.0 : 8.0;
+  const double sweep_us = 2.0 * std::ldexp(bpa, (int)p.n) / 4e12 * 1e6;
+  double us = 0.0;
+  for (const Step& s : p.steps)
+    if (s.kind != Step::FUSED) us += sweep_us;
+  return us;
+}
+
 Plan::~Plan() {
   if (d_passes) (void)hipFree(d_passes);
   if (d_stages) (void)hipFree(d_stages);

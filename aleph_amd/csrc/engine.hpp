@@ -114,6 +114,14 @@ struct Plan {
 std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog, uint32_t n, uint32_t precision,
                                  const PlanOptions& opts);
 
+// Cost-model features of every fused-pass launch of `p` (kPlanFeatures
+// floats per row, launch order); `launch` receives each row's step index.
+constexpr int kPlanFeatures = 16;
+void plan_features(const Plan& p, std::vector<float>& rows, std::vector<uint32_t>& launch);
+// Estimated microseconds of the plan's launches that are not fused passes
+// (each streams the state once at an assumed 4 TB/s).
+double plan_standalone_us(const Plan& p);
+
 // Apply one validated gate with the per-gate kernels (Backend::apply_gate).
 // `scratch` must hold at least 2 * 4^6 doubles of device memory.
 void apply_dense_gate(hipStream_t s, void* state, uint32_t n, uint32_t precision, const DenseGate& g,

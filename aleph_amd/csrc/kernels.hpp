@@ -73,6 +73,20 @@ template <typename T>
 void launch_batch_cut(hipStream_t s, const void* pool, const CutDesc* descs, uint32_t n_inst, uint32_t n_blocks,
                       double* partials, double* out);
 
+// ---- planner cost model (k_mlp.hip) ----
+constexpr int kMlpIn = 16;        // features per candidate pass
+constexpr int kMlpHidden = 64;    // three hidden ReLU layers
+constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + 2 * (kMlpHidden * kMlpHidden + kMlpHidden) +
+                           kMlpHidden + 1;
+// out[i] = model(X[i, :]) for B rows of kMlpIn floats.
+void launch_mlp_forward(hipStream_t s, const float* params, const float* X, uint32_t B, float* out);
+// grads = d/dparams of mean((model(X) - y)^2), loss[0] = that mean (both device, overwritten).
+void launch_mlp_grad(hipStream_t s, const float* params, const float* X, const float* y, uint32_t B, float* grads,
+                     float* loss);
+// One Adam step (t counts from 1).
+void launch_mlp_adam(hipStream_t s, float* params, const float* grads, float* m, float* v, float lr, float b1,
+                     float b2, float eps, uint32_t t);
+
 // ---- readout (k_readout.hip) ----
 // Result layout in `out` (device, 3 doubles): [sum of |a|^2 over bit q clear,
 // sum over bit q set, non-finite count]; qubit < 0 puts everything in [0].  `partials` needs 3 * kReduceBlocks doubles.

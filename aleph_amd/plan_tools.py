@@ -90,6 +90,12 @@ class HostPlan:
             self.terms = [PhaseTerm.from_buffer_copy(s) for s in _array(v.terms, v.n_terms, PhaseTerm,
                                                                          v.term_bytes)]
             self.payload = C.string_at(v.payload, v.payload_bytes) if v.payload_bytes else b""
+            nrows = C.c_uint32()
+            lib.aleph_hip_plan_features(h, None, None, 0, C.byref(nrows))
+            self.features = np.zeros((nrows.value, L.MODEL_INPUTS), dtype=np.float32)
+            self.feature_launch = np.zeros(nrows.value, dtype=np.uint32)
+            lib.aleph_hip_plan_features(h, self.features.ctypes.data_as(L.PF32),
+                                        self.feature_launch.ctypes.data_as(L.PU32), nrows.value, C.byref(nrows))
             st = L.PlanStats()
             lib.aleph_hip_plan_stats_get(h, C.byref(st))
             self.stats = {f: getattr(st, f) for f, _ in L.PlanStats._fields_ if f != "reserved"}

@@ -152,6 +152,30 @@ def qaoa_batch(count: int, seed: int, n_min: int = 12, n_max: int = 24, p: int =
     return graphs, gammas, betas
 
 
+def planner_corpus(count: int, seed: int, n_min: int = 24, n_max: int = 30, d_min: int = 20, d_max: int = 200):
+    """Seeded corpus for the learned planner: n uniform in [n_min, n_max],
+    depth uniform in [d_min, d_max], family cycling brickwall / QFT / HEA
+    (QFT ignores the depth).  Returns a list of (family, n, depth, seed)
+    specs; `corpus_circuit(spec)` builds one."""
+    rng = SplitMix64(seed)
+    out = []
+    for i in range(count):
+        n = n_min + int(rng.next_u64() % (n_max - n_min + 1))
+        d = d_min + int(rng.next_u64() % (d_max - d_min + 1))
+        fam = ("brickwall", "qft", "hea")[int(rng.next_u64() % 3)]
+        out.append((fam, n, d, int(rng.next_u64() & 0xFFFFFFFF)))
+    return out
+
+
+def corpus_circuit(spec) -> Circuit:
+    fam, n, d, s = spec
+    if fam == "brickwall":
+        return brickwall(n, d, seed=s)
+    if fam == "qft":
+        return qft(n)
+    return hea(n, max(1, d // 4), seed=s)
+
+
 # ---------------- OpenQASM 3 subset ----------------
 
 _OPS = {ast.Add: operator.add, ast.Sub: operator.sub, ast.Mult: operator.mul, ast.Div: operator.truediv,

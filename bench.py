@@ -262,12 +262,28 @@ def main():
         "norm_error": norm - 1.0,
     }
     if not args.no_extra:
+        import aleph_amd._lib as L
+        from aleph_amd.cost_model import DEFAULT_PATH, CostModel
         b32 = HipSvBackend(seed=1, precision=32)
         ms32, st32, lm32, k32, norm32, s = time_circuit(b32, circuit, args.steps, args.warmup, **plan_kw)
         s.free()
         log(f"bench: fp32 {ms32:.2f} ms/circuit passes={st32['n_passes']}")
         out["fp32"] = {"ms_per_step": ms32, "passes": st32["n_passes"], "speedup_vs_fp64_rule": ms / ms32,
                        "norm_error": norm32 - 1.0}
+        if DEFAULT_PATH.exists():
+            for prec, bk in ((32, b32), (64, b64)):
+                model = CostModel.load(bk)
+                model.install()
+                msl, stl, _, _, norml, s = time_circuit(bk, circuit, args.steps, args.warmup,
+                                                        planner=L.PLANNER_LEARNED)
+                s.free()
+                log(f"bench: fp{prec} learned {msl:.2f} ms/circuit passes={stl['n_passes']}")
+                out[f"fp{prec}_learned"] = {
+                    "ms_per_step": msl, "passes": stl["n_passes"], "speedup_vs_fp64_rule": ms / msl,
+                    "chosen": {"local_qubits": stl["chosen_local_qubits"], "max_ops": stl["chosen_max_ops"],
+                               "fuse_2q": stl["chosen_flags"] == 0},
+                    "predicted_ms": stl["predicted_us"] / 1e3, "plan_ms": stl["plan_ms"], "norm_error": norml - 1.0}
+                model.free()
         del b32
         q = W.qft(n)
         msq, stq, lmq, kq, normq, sq = time_circuit(b64, q, args.steps, args.warmup, **plan_kw)

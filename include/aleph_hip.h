@@ -194,8 +194,11 @@ typedef struct aleph_hip_plan_stats {
   uint32_t n_stages;        /* register stages summed over passes */
   uint32_t n_ops;           /* device ops after lowering */
   uint32_t n_launches;
-  uint32_t reserved;
+  uint32_t planner_used;    /* ALEPH_HIP_PLANNER_* that produced the plan (LEARNED falls back to RULE without a model) */
   double sweep_bytes;       /* algorithmic HBM bytes over all passes */
+  /* LEARNED: the candidate options the cost model chose and its prediction */
+  uint32_t chosen_local_qubits, chosen_max_ops, chosen_flags, n_candidates;
+  double predicted_us;
 } aleph_hip_plan_stats;
 
 aleph_hip_status aleph_hip_plan_create(aleph_hip_ctx* ctx, uint32_t num_qubits, uint32_t precision,
@@ -251,6 +254,43 @@ aleph_hip_status aleph_hip_expectation(aleph_hip_ctx* ctx, aleph_hip_state* st, 
 /* Marginal over `qubits` (qubits[0] = LSB of the output index), 2^n_qubits doubles. */
 aleph_hip_status aleph_hip_probabilities(aleph_hip_ctx* ctx, aleph_hip_state* st, const uint32_t* qubits,
                                          uint32_t n_qubits, double* out);
+
+/* ---- the learned planner's cost model ----
+ * A perceptron (ALEPH_HIP_MODEL_INPUTS features -> 3 x 64 ReLU -> 1, FP32)
+ * predicting ln(microseconds) of one fused-pass launch from the pass's
+ * features (aleph_hip_plan_features).  It is trained on the device by the
+ * library's own forward, gradient and Adam kernels, on launch times the
+ * engine records (aleph_hip_plan_execute_timed).  With a model set on the
+ * context, ALEPH_HIP_PLANNER_LEARNED plans every candidate (chunk qubits
+ * x pass budget x 4x4 fusion), predicts each plan's time and keeps the
+ * fastest; without one it plans with the rule planner.  No counterpart in
+ * the reference, whose fusion widths are fixed rules
+ * (crates/aleph-ir/src/passes/fuse_kq.rs:185-390, aleph-cuda/src/fusion.rs:48). */
+#define ALEPH_HIP_MODEL_INPUTS 16
+#define ALEPH_HIP_MODEL_PARAMS 9473
+typedef struct aleph_hip_model aleph_hip_model;
+/* params: ALEPH_HIP_MODEL_PARAMS floats, or NULL for a seeded He initialisation. */
+aleph_hip_status aleph_hip_model_create(aleph_hip_ctx* ctx, const float* params, uint64_t seed,
+                                        aleph_hip_model** out);
+aleph_hip_status aleph_hip_model_get_params(aleph_hip_ctx* ctx, const aleph_hip_model* m, float* out);
+aleph_hip_status aleph_hip_model_set_params(aleph_hip_ctx* ctx, aleph_hip_model* m, const float* params);
+/* out[i] = model(X[i]) for `rows` rows of ALEPH_HIP_MODEL_INPUTS floats. */
+aleph_hip_status aleph_hip_model_predict(aleph_hip_ctx* ctx, const aleph_hip_model* m, const float* X,
+                                         uint32_t rows, float* out);
+/* grads = d/dparams of mean((model(X) - y)^2) and *loss that mean (gradient check hook). */
+aleph_hip_status aleph_hip_model_gradient(aleph_hip_ctx* ctx, const aleph_hip_model* m, const float* X,
+                                          const float* y, uint32_t rows, float* grads, float* loss);
+/* Adam on shuffled mini-batches, entirely on the device; *loss = full-data MSE after training. */
+aleph_hip_status aleph_hip_model_train(aleph_hip_ctx* ctx, aleph_hip_model* m, const float* X, const float* y,
+                                       uint32_t rows, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
+                                       float* loss);
+void aleph_hip_model_free(aleph_hip_model* m);
+/* The model LEARNED plans use on this context (copied; NULL clears it). */
+aleph_hip_status aleph_hip_ctx_set_cost_model(aleph_hip_ctx* ctx, const aleph_hip_model* m);
+/* One row of ALEPH_HIP_MODEL_INPUTS features per fused-pass launch, in launch order;
+ * launch[i] receives the row's launch index (for pairing with plan_execute_timed). */
+aleph_hip_status aleph_hip_plan_features(const aleph_hip_plan* plan, float* out, uint32_t* launch,
+                                         uint32_t max_rows, uint32_t* rows);
 
 /* ---- batches: many small circuits of uneven qubit count in one launch ----
  * No counterpart in the reference, which runs a parameter sweep as one

@@ -1,0 +1,121 @@
+"""GPU checks of the learned planner's cost model and of learned plans.
+
+The HIP forward, gradient and Adam kernels are compared with the CPU
+restatement (oracle/mlp_oracle.py, pinned by finite differences); training
+on the device must reduce the loss; every learned plan must equal the
+unfused application of the same circuit (FP64 1e-12, FP32 1e-5); without a
+model the learned planner falls back to the rule planner; and the trained
+model shipped with the package must not lose to the rule planner on
+held-out circuits (measured time).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from aleph_amd import _lib as L
+from aleph_amd import workloads as W
+from aleph_amd.backend import HipSvBackend
+from aleph_amd.cost_model import DEFAULT_PATH, CostModel, record
+from oracle import mlp_oracle as M
+from tests.helpers import PARITY_TOL, max_abs
+
+pytestmark = pytest.mark.gpu
+
+
+def data(seed, rows=300):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(0, 1, (rows, M.IN)).astype(np.float32)
+    y = (np.log(1.0 + 40 * X[:, 2] + 20 * X[:, 5] * X[:, 0]) + 0.1 * rng.normal(size=rows)).astype(np.float32)
+    return X, y
+
+
+def test_forward_and_gradient_match_cpu_oracle(hip):
+    m = CostModel(hip, seed=3)
+    p = m.params().astype(np.float64)
+    X, y = data(0, 257)
+    got = m.predict(X)
+    want = M.forward(p, X.astype(np.float64))
+    assert np.max(np.abs(got - want)) < 1e-4 * max(1.0, np.max(np.abs(want)))
+    loss, g = m.gradient(X, y)
+    wl, wg = M.loss_and_grad(p, X.astype(np.float64), y.astype(np.float64))
+    assert abs(loss - wl) < 1e-4 * max(1.0, wl)
+    assert np.max(np.abs(g - wg)) < 1e-4 * max(1.0, np.max(np.abs(wg)))
+    m.free()
+
+
+def test_one_adam_step_matches_cpu_oracle(hip):
+    m = CostModel(hip, seed=5)
+    p0 = m.params().astype(np.float64)
+    X, y = data(1, 64)
+    m.train(X, y, epochs=1, batch=64, lr=1e-2, seed=0)     # one full batch: one Adam step
+    _, g = M.loss_and_grad(p0, X.astype(np.float64), y.astype(np.float64))
+    want, _, _ = M.adam_step(p0, g, np.zeros_like(p0), np.zeros_like(p0), 1, 1e-2)
+    assert np.max(np.abs(m.params() - want)) < 2e-5
+    m.free()
+
+
+def test_training_on_device_reduces_loss(hip):
+    m = CostModel(hip, seed=7)
+    X, y = data(2, 600)
+    before, _ = m.gradient(X, y)
+    after = m.train(X, y, epochs=60, batch=32, lr=3e-3, seed=1)
+    assert after < 0.25 * before
+
+
+@pytest.mark.parametrize("precision,tol", [(64, PARITY_TOL), (32, 1e-5)])
+def test_learned_plans_equal_unfused(precision, tol):
+    b = HipSvBackend(seed=9, precision=precision)
+    m = CostModel(b, seed=11)
+    m.install()
+    for c in (W.brickwall(14, 10, seed=4), W.qft(13), W.hea(12, 4, seed=2),
+              W.qaoa_maxcut(12, W.random_3_regular(12, 3), [0.4, 1.2], [0.7, 0.3])):
+        psi = W.gaussian_state(c.num_qubits, 5)
+        outs = []
+        for planner in (L.PLANNER_LEARNED, L.PLANNER_UNFUSED):
+            st = b.allocate(c.num_qubits)
+            st.load(psi)
+            plan = b.compile(c, planner=planner)
+            if planner == L.PLANNER_LEARNED:
+                s = plan.stats()
+                assert s["planner_used"] == L.PLANNER_LEARNED and s["n_candidates"] >= 6
+                assert s["chosen_local_qubits"] in (10, 11, 12) and s["chosen_max_ops"] in (12, 24, 48)
+            plan.execute(st)
+            outs.append(st.amplitudes())
+            plan.free()
+            st.free()
+        assert max_abs(outs[0], outs[1]) < tol
+    m.free()
+
+
+def test_learned_without_model_falls_back_to_rule():
+    b = HipSvBackend(seed=1)
+    plan = b.compile(W.brickwall(12, 4, seed=1), planner=L.PLANNER_LEARNED)
+    assert plan.stats()["planner_used"] == L.PLANNER_RULE
+    plan.free()
+
+
+@pytest.mark.skipif(not DEFAULT_PATH.exists(), reason="no trained cost model in the package")
+def test_trained_model_does_not_lose_to_rule_planner():
+    """Held-out circuits (a seed the training corpus did not use): geometric
+    mean of measured rule time / learned time must be >= 0.97, and the
+    figure is printed."""
+    b = HipSvBackend(seed=2, precision=64)
+    m = CostModel.load(b)
+    m.install()
+    specs = W.planner_corpus(8, 999, n_min=22, n_max=24, d_max=80)
+    ratios = []
+    states = {}
+    for spec in specs:
+        c = W.corpus_circuit(spec)
+        n = c.num_qubits
+        states.setdefault(n, b.allocate(n))
+        _, _, rule_ms = record(b, c, states[n], reps=3)
+        plan = b.compile(c, planner=L.PLANNER_LEARNED)
+        learned_ms = min(plan.execute_timed(states[n])[0].sum() for _ in range(3))
+        plan.free()
+        ratios.append(rule_ms / learned_ms)
+    gain = math.exp(np.mean(np.log(ratios)))
+    print(f"learned vs rule on held-out circuits: geometric-mean speedup {gain:.3f} ({np.round(ratios, 3)})")
+    assert gain >= 0.97
