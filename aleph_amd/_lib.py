@@ -57,7 +57,7 @@ class Gate(C.Structure):
 
 class PlanOpts(C.Structure):
     _fields_ = [("planner", u32), ("local_qubits", u32), ("max_ops_per_pass", u32), ("coalesce_bits", u32),
-                ("fuse_1q", u32), ("reg_bits", u32), ("flags", u32), ("reserved", u32)]
+                ("fuse_1q", u32), ("reg_bits", u32), ("flags", u32), ("kq_width", u32)]
 
 
 class PlanStats(C.Structure):

@@ -21,7 +21,7 @@ LIB = PKG / "libaleph_hip.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["capi.cpp", "engine.cpp", "batch.cpp", "k_gates.hip", "k_fused.hip", "k_batch.hip", "k_mlp.hip", "k_readout.hip"]
+SOURCES = ["capi.cpp", "engine.cpp", "batch.cpp", "k_gates.hip", "k_fused.hip", "k_kq_mfma.hip", "k_batch.hip", "k_mlp.hip", "k_readout.hip"]
 
 COMMON = [
     "-O3",

@@ -32,6 +32,7 @@ struct aleph_hip_ctx {
   size_t scratch_bytes = 0;
   double* h_pinned = nullptr;       // 64 doubles of pinned readback
   float* d_model = nullptr;         // cost model of the learned planner (kMlpParams floats) or null
+  std::vector<aleph_hip_state*> states;  // live states, detached from the context when it is destroyed
 };
 
 struct aleph_hip_state {
@@ -293,6 +294,7 @@ void aleph_hip_ctx_destroy(aleph_hip_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->d_scratch) (void)hipFree(c->d_scratch);
   if (c->d_model) (void)hipFree(c->d_model);
+  for (aleph_hip_state* st : c->states) st->ctx = nullptr;  // states may be freed after their context
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -357,6 +359,7 @@ aleph_hip_status aleph_hip_state_alloc(aleph_hip_ctx* c, uint32_t n, uint32_t pr
     if (precision == ALEPH_HIP_F64) launch_init_basis<double>(c->stream, st->d, n);
     else launch_init_basis<float>(c->stream, st->d, n);
     ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->states.push_back(st.get());
     *out = st.release();
   });
 }
@@ -366,6 +369,8 @@ void aleph_hip_state_free(aleph_hip_state* st) {
   if (st->ctx) {
     (void)hipSetDevice(st->ctx->device);
     (void)hipStreamSynchronize(st->ctx->stream);
+    auto& v = st->ctx->states;
+    v.erase(std::remove(v.begin(), v.end(), st), v.end());
   }
   if (st->d) (void)hipFree(st->d);
   delete st;
@@ -523,11 +528,12 @@ aleph_hip_status aleph_hip_apply_kq(aleph_hip_ctx* c, aleph_hip_state* st, uint3
     need(st->n == 64 || (cm >> st->n) == 0, "control mask beyond state");
     for (uint32_t j = 0; j < k; j++) need(!(cm & (1ull << qubits[j])), "control overlaps target");
     const size_t dim = 1ull << k;
+    std::vector<float> f;  // outlives the copy (synchronised below)
     if (st->precision == ALEPH_HIP_F64) {
       ALEPH_HIP_CHECK(hipMemcpyAsync(c->d_scratch, m, 2 * dim * dim * sizeof(double), hipMemcpyHostToDevice, c->stream));
       launch_kq<double>(c->stream, st->d, st->n, k, qubits, cm, c->d_scratch);
     } else {
-      std::vector<float> f(m, m + 2 * dim * dim);
+      f.assign(m, m + 2 * dim * dim);
       ALEPH_HIP_CHECK(hipMemcpyAsync(c->d_scratch, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
       launch_kq<float>(c->stream, st->d, st->n, k, qubits, cm, c->d_scratch);
     }
@@ -655,7 +661,9 @@ std::unique_ptr<aleph_hip_plan> make_plan(aleph_hip_ctx* c, uint32_t n, uint32_t
       po.nontemporal = (opts->flags & ALEPH_HIP_PLAN_NONTEMPORAL) != 0;
       po.fuse_2q = (opts->flags & ALEPH_HIP_PLAN_NO_FUSE_2Q) == 0;
       po.fuse_diag = (opts->flags & ALEPH_HIP_PLAN_NO_FUSE_DIAG) == 0;
+      po.kq_width = opts->kq_width;
     }
+    need(po.kq_width == 0 || (po.kq_width >= 3 && po.kq_width <= kMaxFusedK), "kq_width must be 0 or in [3, 6]");
     need(po.reg_bits == 0 || po.reg_bits == 4 || po.reg_bits == 5, "reg_bits must be 4 or 5");
     need(po.planner <= ALEPH_HIP_PLANNER_LEARNED, "unknown planner");
     need(po.local_qubits == 0 || (po.local_qubits >= (po.reg_bits ? po.reg_bits : (uint32_t)kRegBits) &&

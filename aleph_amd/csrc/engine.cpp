@@ -998,6 +998,128 @@ std::vector<ProgInstr> fuse_diagonal_runs(const std::vector<ProgInstr>& prog, ui
   return out;
 }
 
+namespace {
+
+// Apply gate `g` (targets MSB first, optional controls) to a vector over the
+// block's qubits `bq` (ascending; local bit j is qubit bq[j]).
+void apply_to_local(std::vector<cplx>& v, const std::vector<uint32_t>& bq, const DenseGate& g) {
+  auto local = [&](uint32_t q) {
+    return (uint32_t)(std::lower_bound(bq.begin(), bq.end(), q) - bq.begin());
+  };
+  const uint32_t kg = (uint32_t)g.qubits.size();
+  uint32_t tmask = 0, cmask = 0;
+  std::vector<uint32_t> off(g.dim);
+  for (uint32_t t = 0; t < g.dim; t++) {
+    uint32_t o = 0;
+    for (uint32_t j = 0; j < kg; j++)
+      if ((t >> j) & 1u) o |= 1u << local(g.qubits[kg - 1 - j]);
+    off[t] = o;
+  }
+  for (uint32_t q : g.qubits) tmask |= 1u << local(q);
+  for (uint32_t q : g.controls) cmask |= 1u << local(q);
+  std::vector<cplx> in(g.dim);
+  const uint32_t D = (uint32_t)v.size();
+  for (uint32_t x = 0; x < D; x++) {
+    if ((x & tmask) != 0 || (x & cmask) != cmask) continue;
+    for (uint32_t t = 0; t < g.dim; t++) in[t] = v[x | off[t]];
+    for (uint32_t r = 0; r < g.dim; r++) {
+      cplx acc = 0;
+      for (uint32_t c = 0; c < g.dim; c++) acc += g.m[(size_t)r * g.dim + c] * in[c];
+      v[x | off[r]] = acc;
+    }
+  }
+}
+
+struct OpenBlock {
+  std::vector<uint32_t> qubits;            // ascending
+  std::vector<DenseGate> gates;            // in order
+};
+
+DenseGate block_gate(const OpenBlock& b) {
+  const uint32_t k = (uint32_t)b.qubits.size();
+  const uint32_t D = 1u << k;
+  DenseGate out;
+  out.kind = k == 1 ? ALEPH_HIP_GATE_UNITARY1Q : k == 2 ? ALEPH_HIP_GATE_UNITARY2Q : ALEPH_HIP_GATE_UNITARYKQ;
+  out.qubits.assign(b.qubits.rbegin(), b.qubits.rend());   // MSB first
+  out.dim = D;
+  out.m.assign((size_t)D * D, cplx(0, 0));
+  std::vector<cplx> col(D);
+  for (uint32_t c = 0; c < D; c++) {
+    std::fill(col.begin(), col.end(), cplx(0, 0));
+    col[c] = 1;
+    for (const DenseGate& g : b.gates) apply_to_local(col, b.qubits, g);
+    for (uint32_t r = 0; r < D; r++) out.m[(size_t)r * D + c] = col[r];
+  }
+  out.diagonal = matrix_is_diagonal(out.m, D);
+  return out;
+}
+
+}  // namespace
+
+std::vector<ProgInstr> fuse_dense_blocks(const std::vector<ProgInstr>& prog, uint32_t width) {
+  // Greedy dense blocking: a gate joins the open blocks it touches when
+  // their qubits together with its own number at most `width`; otherwise
+  // those blocks are emitted first.  Open blocks are disjoint, so they
+  // commute and may be emitted in any order.  Blocks of one gate are
+  // emitted unchanged.  (The reference's rule fuser, FuseKq,
+  // crates/aleph-ir/src/passes/fuse_kq.rs:185-390, grows blocks over a
+  // dependency window with its own emit rule; this is a simpler greedy.)
+  std::vector<ProgInstr> out;
+  std::vector<OpenBlock> open;
+  auto emit = [&](size_t i) {
+    ProgInstr pi;
+    pi.type = ProgInstr::GATE;
+    pi.gate = open[i].gates.size() == 1 ? open[i].gates[0] : block_gate(open[i]);
+    out.push_back(std::move(pi));
+    open.erase(open.begin() + (long)i);
+  };
+  auto flush_all = [&] {
+    while (!open.empty()) emit(0);
+  };
+  for (const ProgInstr& pi : prog) {
+    if (pi.type != ProgInstr::GATE) {
+      flush_all();
+      out.push_back(pi);
+      continue;
+    }
+    const DenseGate& g = pi.gate;
+    std::vector<uint32_t> q = g.qubits;
+    q.insert(q.end(), g.controls.begin(), g.controls.end());
+    std::sort(q.begin(), q.end());
+    std::vector<size_t> touch;
+    std::vector<uint32_t> uni = q;
+    for (size_t i = 0; i < open.size(); i++) {
+      bool hit = false;
+      for (uint32_t x : open[i].qubits)
+        if (std::binary_search(q.begin(), q.end(), x)) hit = true;
+      if (hit) {
+        touch.push_back(i);
+        uni.insert(uni.end(), open[i].qubits.begin(), open[i].qubits.end());
+      }
+    }
+    std::sort(uni.begin(), uni.end());
+    uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
+    if (q.size() > width) {
+      for (size_t t = touch.size(); t-- > 0;) emit(touch[t]);
+      out.push_back(pi);
+      continue;
+    }
+    if (uni.size() > width) {
+      for (size_t t = touch.size(); t-- > 0;) emit(touch[t]);
+      open.push_back(OpenBlock{q, {g}});
+      continue;
+    }
+    OpenBlock merged{uni, {}};
+    for (size_t t : touch)
+      merged.gates.insert(merged.gates.end(), open[t].gates.begin(), open[t].gates.end());
+    merged.gates.push_back(g);
+    for (size_t t = touch.size(); t-- > 0;) open.erase(open.begin() + (long)touch[t]);
+    open.push_back(std::move(merged));
+  }
+  flush_all();
+  return out;
+}
+
 std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog, uint32_t n, uint32_t precision,
                                  const PlanOptions& opts) {
   const auto t0 = std::chrono::steady_clock::now();
@@ -1011,7 +1133,9 @@ std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog, uint32_t n,
   if (opts.planner == ALEPH_HIP_PLANNER_UNFUSED || n < G) {
     plan_unfused(*p, prog);
   } else {
-    std::vector<LGate> lg = lower_program(opts.fuse_diag ? fuse_diagonal_runs(prog, n) : prog, n);
+    std::vector<ProgInstr> pre = opts.fuse_diag ? fuse_diagonal_runs(prog, n) : prog;
+    if (opts.kq_width >= 3) pre = fuse_dense_blocks(pre, std::min<uint32_t>(opts.kq_width, kMaxFusedK));
+    std::vector<LGate> lg = lower_program(pre, n);
     if (opts.fuse_1q) lg = fuse_1q_runs(lg);
     if (opts.fuse_2q) lg = fuse_pairs(lg);
     plan_rule(*p, std::move(lg), n, opts);

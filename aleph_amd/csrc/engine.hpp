@@ -67,6 +67,7 @@ struct PlanOptions {
   bool nontemporal = false;
   bool fuse_2q = true;        // fuse 1q neighbours into 2q gates (4x4 blocks)
   bool fuse_diag = true;      // fuse {diagonal, CNOT} runs into one DiagonalPhase
+  uint32_t kq_width = 0;      // >= 3: fuse gates into dense blocks of up to this many qubits (standalone sweeps)
 };
 
 // One device launch of a plan.
@@ -113,6 +114,9 @@ struct Plan {
 
 std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog, uint32_t n, uint32_t precision,
                                  const PlanOptions& opts);
+
+// Greedy fusion of gates into dense blocks of at most `width` qubits.
+std::vector<ProgInstr> fuse_dense_blocks(const std::vector<ProgInstr>& prog, uint32_t width);
 
 // Cost-model features of every fused-pass launch of `p` (kPlanFeatures
 // floats per row, launch order); `launch` receives each row's step index.

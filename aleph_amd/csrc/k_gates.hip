@@ -11,8 +11,13 @@ namespace {
 
 constexpr uint32_t kBlock = 256;
 
+// Blocks for one thread per item, capped so that the dispatch stays under
+// 2^32 work-items (the HSA packet's limit); kernels whose item count can
+// exceed the cap loop grid-stride.
+constexpr uint64_t kMaxBlocks = 1ull << 20;
 inline uint32_t grid_for(uint64_t items) {
   uint64_t g = (items + kBlock - 1) / kBlock;
+  if (g > kMaxBlocks) g = kMaxBlocks;
   return (uint32_t)(g == 0 ? 1 : g);
 }
 
@@ -22,8 +27,7 @@ struct Perm64 { uint32_t p[64]; };
 
 template <typename T>
 __global__ void k_init_basis(C2<T>* __restrict__ s, uint64_t len) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < len) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
     C2<T> v;
     v.x = i == 0 ? (T)1 : (T)0;
     v.y = 0;
@@ -33,46 +37,46 @@ __global__ void k_init_basis(C2<T>* __restrict__ s, uint64_t len) {
 
 template <typename T>
 __global__ void k_1q(C2<T>* __restrict__ s, uint64_t pairs, uint32_t t, uint64_t cm, M8 m) {
-  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= pairs) return;
-  const uint64_t i0 = insert_zero(p, t), i1 = i0 | (1ull << t);
-  if ((i0 & cm) != cm) return;
-  const C2<T> x = s[i0], y = s[i1];
-  const T a = m.v[0], b = m.v[1], c = m.v[2], d = m.v[3], e = m.v[4], f = m.v[5], g = m.v[6], h = m.v[7];
-  C2<T> nx, ny;
-  nx.x = a * x.x - b * x.y + c * y.x - d * y.y;
-  nx.y = a * x.y + b * x.x + c * y.y + d * y.x;
-  ny.x = e * x.x - f * x.y + g * y.x - h * y.y;
-  ny.y = e * x.y + f * x.x + g * y.y + h * y.x;
-  s[i0] = nx;
-  s[i1] = ny;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < pairs; p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i0 = insert_zero(p, t), i1 = i0 | (1ull << t);
+    if ((i0 & cm) != cm) continue;
+    const C2<T> x = s[i0], y = s[i1];
+    const T a = m.v[0], b = m.v[1], c = m.v[2], d = m.v[3], e = m.v[4], f = m.v[5], g = m.v[6], h = m.v[7];
+    C2<T> nx, ny;
+    nx.x = a * x.x - b * x.y + c * y.x - d * y.y;
+    nx.y = a * x.y + b * x.x + c * y.y + d * y.x;
+    ny.x = e * x.x - f * x.y + g * y.x - h * y.y;
+    ny.y = e * x.y + f * x.x + g * y.y + h * y.x;
+    s[i0] = nx;
+    s[i1] = ny;
+  }
 }
 
 template <typename T>
 __global__ void k_2q(C2<T>* __restrict__ s, uint64_t quads, uint32_t qmsb, uint32_t qlsb, uint64_t cm, M32 m) {
-  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= quads) return;
   const uint32_t lo = qmsb < qlsb ? qmsb : qlsb, hi = qmsb < qlsb ? qlsb : qmsb;
-  const uint64_t i = insert_zero(insert_zero(p, lo), hi);
-  if ((i & cm) != cm) return;
-  const uint64_t bm = 1ull << qmsb, bl = 1ull << qlsb;
-  const uint64_t idx[4] = {i, i | bl, i | bm, i | bm | bl};
-  C2<T> v[4];
-#pragma unroll
-  for (int c = 0; c < 4; c++) v[c] = s[idx[c]];
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    T re = 0, im = 0;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const T mr = m.v[2 * (r * 4 + c)], mi = m.v[2 * (r * 4 + c) + 1];
-      re += mr * v[c].x - mi * v[c].y;
-      im += mr * v[c].y + mi * v[c].x;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < quads; p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = insert_zero(insert_zero(p, lo), hi);
+    if ((i & cm) != cm) continue;
+    const uint64_t bm = 1ull << qmsb, bl = 1ull << qlsb;
+    const uint64_t idx[4] = {i, i | bl, i | bm, i | bm | bl};
+    C2<T> v[4];
+  #pragma unroll
+    for (int c = 0; c < 4; c++) v[c] = s[idx[c]];
+  #pragma unroll
+    for (int r = 0; r < 4; r++) {
+      T re = 0, im = 0;
+  #pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const T mr = m.v[2 * (r * 4 + c)], mi = m.v[2 * (r * 4 + c) + 1];
+        re += mr * v[c].x - mi * v[c].y;
+        im += mr * v[c].y + mi * v[c].x;
+      }
+      C2<T> o;
+      o.x = re;
+      o.y = im;
+      s[idx[r]] = o;
     }
-    C2<T> o;
-    o.x = re;
-    o.y = im;
-    s[idx[r]] = o;
   }
 }
 
@@ -129,66 +133,66 @@ __global__ void __launch_bounds__(256) k_kq(C2<T>* __restrict__ s, uint64_t grou
 template <typename T>
 __global__ void k_diag(C2<T>* __restrict__ s, uint64_t len, uint32_t k, Perm64 q, uint64_t cm,
                        const double* __restrict__ d) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= len || (i & cm) != cm) return;
-  uint32_t sel = 0;
-  for (uint32_t j = 0; j < k; j++) sel = (sel << 1) | (uint32_t)((i >> q.p[j]) & 1ull);
-  const T dr = (T)d[2 * sel], di = (T)d[2 * sel + 1];
-  const C2<T> x = s[i];
-  C2<T> y;
-  y.x = x.x * dr - x.y * di;
-  y.y = x.x * di + x.y * dr;
-  s[i] = y;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
+    if ((i & cm) != cm) continue;
+    uint32_t sel = 0;
+    for (uint32_t j = 0; j < k; j++) sel = (sel << 1) | (uint32_t)((i >> q.p[j]) & 1ull);
+    const T dr = (T)d[2 * sel], di = (T)d[2 * sel + 1];
+    const C2<T> x = s[i];
+    C2<T> y;
+    y.x = x.x * dr - x.y * di;
+    y.y = x.x * di + x.y * dr;
+    s[i] = y;
+  }
 }
 
 template <typename T>
 __global__ void k_swap(C2<T>* __restrict__ s, uint64_t quads, uint32_t a, uint32_t b, uint64_t cm) {
-  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= quads) return;
   const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
-  const uint64_t i = insert_zero(insert_zero(p, lo), hi);
-  if ((i & cm) != cm) return;
-  const uint64_t x = i | (1ull << a), y = i | (1ull << b);
-  const C2<T> t = s[x];
-  s[x] = s[y];
-  s[y] = t;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < quads; p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = insert_zero(insert_zero(p, lo), hi);
+    if ((i & cm) != cm) continue;
+    const uint64_t x = i | (1ull << a), y = i | (1ull << b);
+    const C2<T> t = s[x];
+    s[x] = s[y];
+    s[y] = t;
+  }
 }
 
 template <typename T>
 __global__ void k_phase_poly(C2<T>* __restrict__ s, uint64_t len, uint32_t nt, const double* __restrict__ ang,
                              const uint64_t* __restrict__ conds, const uint32_t* __restrict__ off) {
-  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= len) return;
-  double phi = 0.0;
-  for (uint32_t t = 0; t < nt; t++) {
-    bool all = true;
-    for (uint32_t c = off[t]; c < off[t + 1]; c++)
-      if ((__popcll(conds[c] & x) & 1) == 0) { all = false; break; }
-    if (all) phi += ang[t];
+  for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < len; x += (uint64_t)gridDim.x * blockDim.x) {
+    double phi = 0.0;
+    for (uint32_t t = 0; t < nt; t++) {
+      bool all = true;
+      for (uint32_t c = off[t]; c < off[t + 1]; c++)
+        if ((__popcll(conds[c] & x) & 1) == 0) { all = false; break; }
+      if (all) phi += ang[t];
+    }
+    double sn, cs;
+    sincos(phi, &sn, &cs);
+    const C2<T> v = s[x];
+    C2<T> o;
+    o.x = v.x * (T)cs - v.y * (T)sn;
+    o.y = v.x * (T)sn + v.y * (T)cs;
+    s[x] = o;
   }
-  double sn, cs;
-  sincos(phi, &sn, &cs);
-  const C2<T> v = s[x];
-  C2<T> o;
-  o.x = v.x * (T)cs - v.y * (T)sn;
-  o.y = v.x * (T)sn + v.y * (T)cs;
-  s[x] = o;
 }
 
 template <typename T>
 __global__ void k_permute(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, uint64_t len, uint32_t n,
                           Perm64 perm) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= len) return;
-  uint64_t j = 0;
-  for (uint32_t l = 0; l < n; l++)
-    if ((i >> l) & 1ull) j |= 1ull << perm.p[l];
-  dst[i] = src[j];
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t j = 0;
+    for (uint32_t l = 0; l < n; l++)
+      if ((i >> l) & 1ull) j |= 1ull << perm.p[l];
+    dst[i] = src[j];
+  }
 }
 
 __global__ void k_widen(const C2<float>* __restrict__ a, C2<double>* __restrict__ b, uint64_t len) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < len) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
     C2<double> v;
     v.x = a[i].x;
     v.y = a[i].y;
@@ -197,8 +201,7 @@ __global__ void k_widen(const C2<float>* __restrict__ a, C2<double>* __restrict_
 }
 
 __global__ void k_narrow(const C2<double>* __restrict__ a, C2<float>* __restrict__ b, uint64_t len) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < len) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
     C2<float> v;
     v.x = (float)a[i].x;
     v.y = (float)a[i].y;
@@ -242,6 +245,10 @@ void launch_2q(hipStream_t st, void* state, uint32_t n, uint32_t qm, uint32_t ql
 template <typename T>
 void launch_kq(hipStream_t st, void* state, uint32_t n, uint32_t k, const uint32_t* qubits, uint64_t cmask,
                const void* dmat) {
+  if (k >= 4 && k <= 6 && n >= k + 4) {
+    launch_kq_mfma<T>(st, state, n, k, qubits, cmask, dmat);
+    return;
+  }
   const uint32_t dim = 1u << k;
   const uint64_t groups = 1ull << (n - k);
   const uint32_t per_tile = kBlock / dim;

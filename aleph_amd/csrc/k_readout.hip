@@ -96,18 +96,18 @@ __global__ void __launch_bounds__(kThreads) k_reduce_final(const double* __restr
 
 template <typename T>
 __global__ void k_collapse(C2<T>* __restrict__ s, uint64_t len, uint64_t qmask, int32_t outcome, double scale) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= len) return;
-  const bool keep = ((i & qmask) != 0) == (outcome != 0);
-  C2<T> a = s[i];
-  if (keep) {
-    a.x = (T)((double)a.x * scale);
-    a.y = (T)((double)a.y * scale);
-  } else {
-    a.x = 0;
-    a.y = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
+    const bool keep = ((i & qmask) != 0) == (outcome != 0);
+    C2<T> a = s[i];
+    if (keep) {
+      a.x = (T)((double)a.x * scale);
+      a.y = (T)((double)a.y * scale);
+    } else {
+      a.x = 0;
+      a.y = 0;
+    }
+    s[i] = a;
   }
-  s[i] = a;
 }
 
 template <typename T>
@@ -159,8 +159,8 @@ __global__ void __launch_bounds__(kThreads) k_marginal(const C2<T>* __restrict__
 }
 
 __global__ void k_zero(double* __restrict__ p, uint64_t count) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) p[i] = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
 }
 
 // ---- sampler ----
@@ -228,70 +228,74 @@ __global__ void __launch_bounds__(kThreads) k_sample(const C2<T>* __restrict__ s
                                                      const double* __restrict__ uni, uint32_t shots,
                                                      uint64_t* __restrict__ out) {
   const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t shot = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (shot >= shots) return;
-  const double target = uni[shot] * total[0];
-  // last block whose prefix <= target among blocks with positive mass
-  uint64_t lo = 0, hi = nblocks;  // invariant: answer in [lo, hi)
-  while (hi - lo > 1) {
-    const uint64_t mid = (lo + hi) >> 1;
-    if (prefix[mid] <= target) lo = mid;
-    else hi = mid;
-  }
-  uint64_t blk = lo;
-  while (blk > 0 && sums[blk] <= 0.0) blk--;          // never land in an empty block
-  while (blk + 1 < nblocks && sums[blk] <= 0.0) blk++;
-  const double local = target - prefix[blk];
-  const uint64_t base = blk * kSampleBlock;
-  const uint32_t per = kSampleBlock / 64;             // amplitudes per lane
-  double mine = 0.0;
-  for (uint32_t j = 0; j < per; j++) {
-    const uint64_t i = base + (uint64_t)lane * per + j;
-    if (i < len) {
-      const C2<T> a = s[i];
-      mine += (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+  const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t shot = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; shot < shots; shot += waves) {
+    const double target = uni[shot] * total[0];
+    // last block whose prefix <= target among blocks with positive mass
+    uint64_t lo = 0, hi = nblocks;  // invariant: answer in [lo, hi)
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (prefix[mid] <= target) lo = mid;
+      else hi = mid;
     }
-  }
-  // inclusive scan over lanes
-  double incl = mine;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const double o = __shfl_up(incl, off, 64);
-    if ((int)lane >= off) incl += o;
-  }
-  const double excl = incl - mine;
-  const bool hit = mine > 0.0 && local < incl && local >= excl;
-  const uint64_t hits = __ballot(hit);
-  const uint64_t pos_lanes = __ballot(mine > 0.0);
-  int who;
-  if (hits) who = __ffsll((long long)hits) - 1;
-  else who = pos_lanes ? 63 - __clzll((long long)pos_lanes) : 0;   // rounding overflow: last positive lane
-  if ((int)lane == who) {
-    const double rem = hits ? local - excl : 1e308;
-    double cum = 0.0;
-    uint64_t pick = base + (uint64_t)lane * per;
-    uint64_t last_pos = pick;
-    bool found = false;
+    uint64_t blk = lo;
+    while (blk > 0 && sums[blk] <= 0.0) blk--;          // never land in an empty block
+    while (blk + 1 < nblocks && sums[blk] <= 0.0) blk++;
+    const double local = target - prefix[blk];
+    const uint64_t base = blk * kSampleBlock;
+    const uint32_t per = kSampleBlock / 64;             // amplitudes per lane
+    double mine = 0.0;
     for (uint32_t j = 0; j < per; j++) {
       const uint64_t i = base + (uint64_t)lane * per + j;
-      if (i >= len) break;
-      const C2<T> a = s[i];
-      const double p = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
-      if (p > 0.0) {
-        last_pos = i;
-        cum += p;
-        if (!found && rem < cum) {
-          pick = i;
-          found = true;
-        }
+      if (i < len) {
+        const C2<T> a = s[i];
+        mine += (double)a.x * (double)a.x + (double)a.y * (double)a.y;
       }
     }
-    out[shot] = found ? pick : last_pos;
+    // inclusive scan over lanes
+    double incl = mine;
+  #pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const double o = __shfl_up(incl, off, 64);
+      if ((int)lane >= off) incl += o;
+    }
+    const double excl = incl - mine;
+    const bool hit = mine > 0.0 && local < incl && local >= excl;
+    const uint64_t hits = __ballot(hit);
+    const uint64_t pos_lanes = __ballot(mine > 0.0);
+    int who;
+    if (hits) who = __ffsll((long long)hits) - 1;
+    else who = pos_lanes ? 63 - __clzll((long long)pos_lanes) : 0;   // rounding overflow: last positive lane
+    if ((int)lane == who) {
+      const double rem = hits ? local - excl : 1e308;
+      double cum = 0.0;
+      uint64_t pick = base + (uint64_t)lane * per;
+      uint64_t last_pos = pick;
+      bool found = false;
+      for (uint32_t j = 0; j < per; j++) {
+        const uint64_t i = base + (uint64_t)lane * per + j;
+        if (i >= len) break;
+        const C2<T> a = s[i];
+        const double p = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+        if (p > 0.0) {
+          last_pos = i;
+          cum += p;
+          if (!found && rem < cum) {
+            pick = i;
+            found = true;
+          }
+        }
+      }
+      out[shot] = found ? pick : last_pos;
+    }
   }
 }
 
+// Capped so that a dispatch stays under 2^32 work-items; kernels that can
+// exceed the cap loop grid-stride (k_sample's grid is shots * 64 threads).
 inline uint32_t grid_for(uint64_t items) {
-  const uint64_t g = (items + kThreads - 1) / kThreads;
+  uint64_t g = (items + kThreads - 1) / kThreads;
+  if (g > (1ull << 20)) g = 1ull << 20;
   return (uint32_t)(g == 0 ? 1 : g);
 }
 

@@ -23,6 +23,11 @@ template <typename T>
 void launch_kq(hipStream_t s, void* state, uint32_t n, uint32_t k, const uint32_t* qubits, uint64_t cmask,
                const void* dmat);
 // Diagonal on k qubits (MSB first); `ddiag` device copy of 2^k complex in double.
+// The same for k in [4, 6] on the matrix cores (k_kq_mfma.hip); launch_kq
+// routes those widths here.
+template <typename T>
+void launch_kq_mfma(hipStream_t s, void* state, uint32_t n, uint32_t k, const uint32_t* qubits, uint64_t cmask,
+                    const void* dmat);
 template <typename T>
 void launch_diag(hipStream_t s, void* state, uint32_t n, uint32_t k, const uint32_t* qubits, uint64_t cmask,
                  const double* ddiag);

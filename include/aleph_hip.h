@@ -179,7 +179,8 @@ typedef struct aleph_hip_plan_opts {
   uint32_t fuse_1q;         /* merge adjacent 1q gates on one qubit before planning */
   uint32_t reg_bits;        /* amplitudes per thread = 2^reg_bits, 4 or 5 (0 = default 4) */
   uint32_t flags;           /* ALEPH_HIP_PLAN_* bits */
-  uint32_t reserved;
+  uint32_t kq_width;        /* 3..6: fuse gates into dense blocks of up to this many qubits, each one
+                               sweep (widths >= 4 on the matrix cores); 0 = off */
 } aleph_hip_plan_opts;
 enum {
   ALEPH_HIP_PLAN_NONTEMPORAL = 1, /* stream chunks with non-temporal loads/stores */

@@ -71,3 +71,16 @@ def test_program_encoding_roundtrip():
     w, r = c2.encode()
     assert list(w[:7]) == [25, 3, 0, 3, 0, 1, 2]
     assert r.size == 2 * 64 + 1
+
+
+@pytest.mark.gpu
+def test_state_may_outlive_its_context():
+    """Freeing a state after its context is destroyed must not touch the
+    context (teardown order at interpreter exit is arbitrary)."""
+    from aleph_amd.backend import HipSvBackend
+    if L.device_count() == 0:
+        pytest.skip("no HIP device")
+    b = HipSvBackend(seed=1)
+    st = b.allocate(6)
+    b.close()
+    st.free()

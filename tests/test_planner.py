@@ -169,3 +169,30 @@ def test_emulated_fp32_plans(oracle_lib, name):
     psi = W.gaussian_state(c.num_qubits, 1)
     plan = HostPlan(c, precision=32)
     assert max_abs(emulate(plan, psi, precision=32), oracle_state(oracle_lib, c, psi)) < 1e-5
+
+
+@pytest.mark.parametrize("width", [3, 4, 5, 6])
+@pytest.mark.parametrize("name", ["brickwall", "random", "qft"])
+def test_dense_block_fusion_matches_oracle(oracle_lib, width, name):
+    """Greedy dense blocks of up to `width` qubits (standalone sweeps): the
+    emulated plan equals the oracle's gate-by-gate run at 1e-12."""
+    n = 9
+    if name == "brickwall":
+        c = W.brickwall(n, 10, seed=6)
+    elif name == "qft":
+        c = W.qft(n)
+    else:
+        c = random_circuit(n, 70, np.random.default_rng(width), kq=False)
+    psi = W.gaussian_state(n, width)
+    plan = HostPlan(c, kq_width=width)
+    ks = [s.k for s in plan.steps if s.kind == 1]
+    assert ks and max(ks) <= width
+    assert max_abs(emulate(plan, psi), oracle_state(oracle_lib, c, psi)) < 1e-12
+    if width >= 4:
+        assert max(ks) >= 4
+
+
+def test_dense_block_count_falls_with_width():
+    c = W.brickwall(16, 20, seed=1)
+    counts = [sum(1 for s in HostPlan(c, kq_width=w).steps if s.kind == 1) for w in (3, 4, 5, 6)]
+    assert counts[0] > counts[1] > counts[2] >= counts[3], counts
