@@ -74,32 +74,47 @@ def time_circuit(backend, circuit, steps: int, warmup: int, **plan_kw):
 
 
 def cpu_baseline(circuit, budget_s: float):
-    """Time the CPU oracle gate by gate on the first gates of the same circuit
-    until the budget is spent; scale the per-gate rate to the whole circuit."""
+    """The reference's CPU final-state path restated (oracle/pipeline.py:
+    Fuse1qRuns, Fuse2q, FuseKq(4), TileBlock(15), then the oracle's
+    OpenMP kernels per instruction), timed on this host's cores on a
+    bounded sample: instructions run in program order until the budget is
+    spent, and the whole list is extrapolated from the mean time of each
+    instruction kind (kind = gate name and width, or tiled block)."""
     from oracle import oracle as orc
+    from oracle import pipeline as P
     threads = os.cpu_count() or 1
     orc.set_threads(threads)
+    instrs = P.cpu_pipeline(circuit)
+
+    def key(i):
+        return "tiled" if isinstance(i, P.Tiled) else f"{i.gate.name}/{len(i.qubits)}"
+
     o = orc.OracleSv()
     t_alloc = time.perf_counter()
-    st = o.allocate(circuit.num_qubits) if circuit.num_qubits <= 28 else orc.OracleState(circuit.num_qubits, cap=64)
+    st = orc.OracleState(circuit.num_qubits, cap=64)
     alloc_s = time.perf_counter() - t_alloc
-    gates = circuit.gates()
+    times = {}
     done = 0
     t0 = time.perf_counter()
-    while done < len(gates) and time.perf_counter() - t0 < budget_s:
-        o.apply_gate(st, gates[done])
+    while done < len(instrs) and time.perf_counter() - t0 < budget_s:
+        t = time.perf_counter()
+        P.apply(o, st, [instrs[done]])
+        times.setdefault(key(instrs[done]), []).append(time.perf_counter() - t)
         done += 1
     el = time.perf_counter() - t0
-    per_gate = el / max(done, 1)
+    overall = el / max(done, 1)
+    est = sum(float(np.mean(times[key(i)])) if key(i) in times else overall for i in instrs)
     del st
+    kinds = {k: len(v) for k, v in times.items()}
     return {
-        "value": per_gate * len(gates),
+        "value": est,
         "unit": "s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"first {done} of {len(gates)} gates of the same n={circuit.num_qubits} circuit, "
-                   f"one full-state sweep per gate (per-gate NaiveSvBackend path, no IR fusion), "
-                   f"{el:.2f} s measured, scaled linearly; allocation {alloc_s:.2f} s not included"),
+        "sample": (f"reference CPU final-state pipeline restated (Fuse1qRuns, Fuse2q, FuseKq(4), TileBlock(15)): "
+                   f"{len(circuit.gates())} gates -> {len(instrs)} instructions; the first {done} run on the same "
+                   f"n={circuit.num_qubits} state in {el:.1f} s ({kinds}), whole list extrapolated per "
+                   f"instruction kind; allocation {alloc_s:.1f} s not included"),
     }
 
 
