@@ -18,11 +18,20 @@ namespace fused {
 
 // XOR-fold swizzle of the chunk-local index into LDS slots: bits >= SW are
 // folded into the low SW bits, so lanes whose thread bits map to high local
-// bits still spread over distinct 128-byte bank windows.
+// bits still spread over distinct 128-byte bank windows.  It is linear over
+// XOR: swz(a ^ b) == swz(a) ^ swz(b).
 template <int SW>
 __device__ __forceinline__ uint32_t swz(uint32_t li) {
   const uint32_t h = li >> SW;
   return li ^ ((h ^ (h >> SW) ^ (h >> (2 * SW)) ^ (h >> (3 * SW))) & ((1u << SW) - 1u));
+}
+
+// x with its sign bit XORed by (s & t), s in {0, 0x80000000}.
+__device__ __forceinline__ double flip_sign(double x, uint32_t s, uint32_t t) {
+  return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, x) ^ ((uint64_t)(s & t) << 32));
+}
+__device__ __forceinline__ float flip_sign(float x, uint32_t s, uint32_t t) {
+  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) ^ (s & t));
 }
 
 template <typename T>
@@ -176,14 +185,16 @@ __device__ __forceinline__ void op_diag(C2<T> (&a)[1 << G], const DevOp* __restr
 
 template <typename T, int G>
 __device__ __forceinline__ void op_sign(C2<T> (&a)[1 << G], const DevOp* __restrict__ op, uint64_t tgo) {
+  // Sign-bit XOR with (thread condition as a bit mask) & (uniform slot
+  // condition): no divergent branch, one bitop per word.
   const uint64_t mo = op->cmask;
   const uint32_t rm = op->a;
-  const bool th = (tgo & mo) == mo;
+  const uint32_t tm = ((tgo & mo) == mo) ? ~0u : 0u;
 #pragma unroll
   for (int r = 0; r < (1 << G); r++) {
-    const bool flip = th && ((uint32_t)r & rm) == rm;
-    a[r].x = flip ? -a[r].x : a[r].x;
-    a[r].y = flip ? -a[r].y : a[r].y;
+    const uint32_t sr = (((uint32_t)r & rm) == rm) ? 0x80000000u : 0u;   // uniform
+    a[r].x = flip_sign(a[r].x, sr, tm);
+    a[r].y = flip_sign(a[r].y, sr, tm);
   }
 }
 
@@ -335,17 +346,23 @@ __device__ __forceinline__ void fused_chunk(C2<T>* __restrict__ state, const Pas
       rg[g] = sd->rg[g];
       rl[g] = sd->rl[g];
     }
+    // Thread and register bits are disjoint, so the global index is the
+    // thread's base plus a uniform register offset, and (swz being linear
+    // over XOR) the LDS slot is swz(thread part) ^ swz(register part), the
+    // latter uniform: one vector op per access instead of the whole fold.
+    C2<T>* const tp = state + tgo;
+    const uint32_t ts = swz<SW>(tlo);
     if (active) {
       if (s == 0) {
 #pragma unroll
         for (int r = 0; r < N; r++) {
-          const V v = gload<NT>(reinterpret_cast<const V*>(state + (tgo | roff<G>(rg, r))));
+          const V v = gload<NT>(reinterpret_cast<const V*>(tp + roff<G>(rg, r)));
           a[r].x = v.x;
           a[r].y = v.y;
         }
       } else {
 #pragma unroll
-        for (int r = 0; r < N; r++) a[r] = lds[swz<SW>(tlo | loff<G>(rl, r))];
+        for (int r = 0; r < N; r++) a[r] = lds[ts ^ swz<SW>(loff<G>(rl, r))];
       }
       run_ops<T, G, FEAT>(a, ops, terms, sd->op_begin, sd->op_end, tgo, rg);
     }
@@ -356,14 +373,14 @@ __device__ __forceinline__ void fused_chunk(C2<T>* __restrict__ state, const Pas
           V v;
           v.x = a[r].x;
           v.y = a[r].y;
-          gstore<NT>(reinterpret_cast<V*>(state + (tgo | roff<G>(rg, r))), v);
+          gstore<NT>(reinterpret_cast<V*>(tp + roff<G>(rg, r)), v);
         }
       }
     } else {
       if (s > 0) __syncthreads();
       if (active) {
 #pragma unroll
-        for (int r = 0; r < N; r++) lds[swz<SW>(tlo | loff<G>(rl, r))] = a[r];
+        for (int r = 0; r < N; r++) lds[ts ^ swz<SW>(loff<G>(rl, r))] = a[r];
       }
       __syncthreads();
     }

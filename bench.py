@@ -23,6 +23,22 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64, vector and matrix alike
+FMA_PER_AMP = {1: 8, 2: 16, 3: 4}  # OP_1Q (2x2), OP_2Q (4x4), OP_DIAG: real FMAs per amplitude
+
+
+def fused_pass_flops(circuit, precision: int) -> float:
+    """Algorithmic FLOPs (2 per real FMA) of the plan's fused passes, averaged
+    per launch: a 2x2 complex matrix costs 2 complex multiply-adds = 8 real
+    FMAs per amplitude, a 4x4 one 16, a diagonal 4; a controlled op counts
+    only the fraction 2^-c of amplitudes its c controls select; sign flips
+    and phase polynomials are counted as zero."""
+    from aleph_amd.plan_tools import HostPlan
+    hp = HostPlan(circuit, precision=precision)
+    fma = 0.0
+    for o in hp.ops:
+        fma += FMA_PER_AMP.get(o.type, 0) * 0.5 ** bin(o.cmask).count("1")
+    return 2.0 * fma * float(1 << circuit.num_qubits) / max(1, len(hp.passes))
 
 
 def parse():
@@ -251,6 +267,7 @@ def main():
     avg_ms = float(fused.mean()) if fused.size else float("nan")
     achieved = sweep_bytes / (avg_ms * 1e-3) / 1e9 if fused.size else 0.0
     total_bytes = stats["sweep_bytes"]
+    flops = fused_pass_flops(circuit, 64)
     out = {
         "metric": "n=30 FP64 circuit s + HBM GB/s %peak; FP32+learned vs FP64 rule speedup",
         "value": ms / 1e3,
@@ -273,7 +290,13 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_fused_pass<double>", "avg_launch_ms": avg_ms, "launches": int(fused.size),
-                     "algorithmic_bytes_per_launch": sweep_bytes},
+                     "algorithmic_bytes_per_launch": sweep_bytes,
+                     "fp64": {"algorithmic_flops_per_launch": flops,
+                              "achieved_tflops": flops / (avg_ms * 1e-3) / 1e12 if fused.size else 0.0,
+                              "peak_tflops": FP64_PEAK_TFLOPS,
+                              "frac": flops / (avg_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS if fused.size else 0.0,
+                              "hbm_floor_ms": sweep_bytes / (HBM_PEAK_GBS * 1e9) * 1e3,
+                              "fp64_floor_ms": flops / (FP64_PEAK_TFLOPS * 1e12) * 1e3}},
         "norm_error": norm - 1.0,
     }
     if not args.no_extra:
