@@ -42,6 +42,7 @@ PLANNER_UNFUSED, PLANNER_RULE, PLANNER_LEARNED = 0, 1, 2
 PLAN_NONTEMPORAL = 1
 PLAN_NO_FUSE_2Q = 2
 PLAN_NO_FUSE_DIAG = 4
+PLAN_JIT = 8
 INSTR_DIAGONAL_PHASE, INSTR_BARRIER = 64, 65
 
 
@@ -169,6 +170,7 @@ SIGNATURES = {
     "aleph_hip_model_free": (None, [P]),
     "aleph_hip_ctx_set_cost_model": (i32, [P, P]),
     "aleph_hip_plan_features": (i32, [P, PF32, PU32, u32, PU32]),
+    "aleph_hip_plan_jit_source": (i32, [P, u32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "aleph_hip_batch_build_host_qaoa": (i32, [u32, PU32, PU64, PU32, u32, PF64, PF64, u32, C.POINTER(BatchOpts),
                                               C.POINTER(P), C.POINTER(Error)]),
     "aleph_hip_batch_view_get": (i32, [P, C.POINTER(BatchView)]),

@@ -315,14 +315,14 @@ class HipSvBackend:
     # ---- plans ----
     def compile(self, circuit: Circuit, planner: int = L.PLANNER_RULE, local_qubits: int = 0, max_ops: int = 0,
                 coalesce_bits: int = 0, fuse_1q: bool = True, reg_bits: int = 0, nontemporal: bool = False,
-                fuse_2q: bool = True, fuse_diag: bool = True, kq_width: int = 0) -> HipPlan:
+                fuse_2q: bool = True, fuse_diag: bool = True, kq_width: int = 0, jit: bool = False) -> HipPlan:
         words, reals = circuit.encode()
         nw, nr = int(words.size), int(reals.size)
         words = np.ascontiguousarray(words if nw else np.zeros(1, np.uint32))
         reals = np.ascontiguousarray(reals if nr else np.zeros(1, np.float64))
         opts = L.PlanOpts(planner, local_qubits, max_ops, coalesce_bits, int(fuse_1q), reg_bits,
                           (L.PLAN_NONTEMPORAL if nontemporal else 0) | (0 if fuse_2q else L.PLAN_NO_FUSE_2Q)
-                          | (0 if fuse_diag else L.PLAN_NO_FUSE_DIAG), kq_width)
+                          | (0 if fuse_diag else L.PLAN_NO_FUSE_DIAG) | (L.PLAN_JIT if jit else 0), kq_width)
         h = C.c_void_p()
         self._check(L.load().aleph_hip_plan_create(self._ctx, circuit.num_qubits, self.precision,
                                                    _ptr(words, C.c_uint32), nw, _ptr(reals, C.c_double), nr,

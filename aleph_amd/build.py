@@ -21,7 +21,7 @@ LIB = PKG / "libaleph_hip.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["capi.cpp", "engine.cpp", "batch.cpp", "k_gates.hip", "k_fused.hip", "k_kq_mfma.hip", "k_batch.hip", "k_mlp.hip", "k_readout.hip"]
+SOURCES = ["capi.cpp", "engine.cpp", "batch.cpp", "jit.cpp", "k_gates.hip", "k_fused.hip", "k_kq_mfma.hip", "k_batch.hip", "k_mlp.hip", "k_readout.hip"]
 
 COMMON = [
     "-O3",
@@ -39,7 +39,7 @@ COMMON = [
 
 def _compile(src: str) -> tuple[str, str]:
     out = BUILD / (src + ".o")
-    cmd = [HIPCC, *COMMON, "-I", str(PKG.parent / "include"), "-c", str(CSRC / src), "-o", str(out)]
+    cmd = [HIPCC, *COMMON, "-I", str(PKG.parent / "include"), "-I", str(BUILD), "-c", str(CSRC / src), "-o", str(out)]
     if src.endswith(".cpp"):
         cmd[1:1] = ["-x", "hip"]
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -48,8 +48,31 @@ def _compile(src: str) -> tuple[str, str]:
     return str(out), res.stderr
 
 
+# Device headers the generated pass kernels compile against (hiprtc sees
+# only these), embedded into the library as string literals.
+JIT_HEADERS = ["device_types.hpp", "plan.hpp", "fused_chunk.hpp"]
+
+
+def _write_jit_headers() -> None:
+    delim = "ALEPH_JIT_HDR"
+    lines = [f"static const int kJitHeaderCount = {len(JIT_HEADERS)};",
+             "static const char* kJitHeaderNames[] = {" + ", ".join(f'"{h}"' for h in JIT_HEADERS) + "};",
+             "static const char* kJitHeaderSources[] = {"]
+    for h in JIT_HEADERS:
+        text = (CSRC / h).read_text()
+        if f"){delim}" in text:
+            raise RuntimeError(f"{h} contains the raw-string delimiter")
+        lines.append(f'R"{delim}({text}){delim}",')
+    lines.append("};")
+    out = BUILD / "jit_headers.inc"
+    new = "\n".join(lines) + "\n"
+    if not out.exists() or out.read_text() != new:
+        out.write_text(new)
+
+
 def build(verbose: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
+    _write_jit_headers()
     with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
         results = list(ex.map(_compile, SOURCES))
     objs = [r[0] for r in results]
@@ -58,7 +81,7 @@ def build(verbose: bool = False) -> Path:
             if err.strip():
                 print(err, file=sys.stderr)
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs, "-lhiprtc"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")

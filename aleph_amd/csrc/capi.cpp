@@ -18,6 +18,7 @@
 #include "common.hpp"
 #include "engine.hpp"
 #include "gates.hpp"
+#include "jit.hpp"
 #include "kernels.hpp"
 
 using namespace aleph;
@@ -659,6 +660,7 @@ std::unique_ptr<aleph_hip_plan> make_plan(aleph_hip_ctx* c, uint32_t n, uint32_t
       po.fuse_1q = opts->fuse_1q != 0;
       po.reg_bits = opts->reg_bits;
       po.nontemporal = (opts->flags & ALEPH_HIP_PLAN_NONTEMPORAL) != 0;
+      po.jit = (opts->flags & ALEPH_HIP_PLAN_JIT) != 0;
       po.fuse_2q = (opts->flags & ALEPH_HIP_PLAN_NO_FUSE_2Q) == 0;
       po.fuse_diag = (opts->flags & ALEPH_HIP_PLAN_NO_FUSE_DIAG) == 0;
       po.kq_width = opts->kq_width;
@@ -691,6 +693,7 @@ aleph_hip_status aleph_hip_plan_create(aleph_hip_ctx* c, uint32_t n, uint32_t pr
     *out = nullptr;
     auto h = make_plan(c, n, precision, words, n_words, reals, n_reals, opts);
     h->plan->upload(c->stream);
+    if (h->plan->opts.jit) jit_compile_plan(*h->plan);
     *out = h.release();
   });
 }
@@ -1401,6 +1404,23 @@ aleph_hip_status aleph_hip_plan_features(const aleph_hip_plan* plan, float* out,
   if (out) std::copy(f.begin(), f.begin() + (size_t)n * kPlanFeatures, out);
   if (launch) std::copy(l.begin(), l.begin() + n, launch);
   return ALEPH_HIP_OK;
+}
+
+aleph_hip_status aleph_hip_plan_jit_source(const aleph_hip_plan* plan, uint32_t pass, char* out, size_t cap,
+                                           size_t* len) {
+  if (!plan || !plan->plan || !len || pass >= plan->plan->passes.size()) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  try {
+    const std::string s = jit_pass_source(*plan->plan, pass);
+    *len = s.size();
+    if (out && cap) {
+      const size_t n = std::min(cap - 1, s.size());
+      std::memcpy(out, s.data(), n);
+      out[n] = 0;
+    }
+    return ALEPH_HIP_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
 }
 
 }  // extern "C"

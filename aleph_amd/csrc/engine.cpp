@@ -14,6 +14,7 @@
 #include <cstring>
 #include <map>
 
+#include "jit.hpp"
 #include "kernels.hpp"
 
 namespace aleph {
@@ -1241,6 +1242,10 @@ static void execute_t(const Plan& p, hipStream_t s, void* state, hipEvent_t* ev)
     if (ev) ALEPH_HIP_CHECK(hipEventRecord(ev[i], s));
     switch (st.kind) {
       case Step::FUSED:
+        if (!p.jit.empty() && p.jit[st.pass]) {
+          launch_jit_pass(s, p, i, state);
+          break;
+        }
         launch_fused_pass<T>(s, state, p.n, st.m, st.n_stages, p.passes[st.pass].features,
                              p.passes[st.pass].reg_bits, p.d_passes + st.pass, p.d_stages, p.d_ops, p.d_terms);
         break;

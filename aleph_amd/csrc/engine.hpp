@@ -65,6 +65,7 @@ struct PlanOptions {
   bool fuse_1q = true;
   uint32_t reg_bits = 0;      // 0: default (4)
   bool nontemporal = false;
+  bool jit = false;           // generate and compile one kernel per fused pass at plan creation
   bool fuse_2q = true;        // fuse 1q neighbours into 2q gates (4x4 blocks)
   bool fuse_diag = true;      // fuse {diagonal, CNOT} runs into one DiagonalPhase
   uint32_t kq_width = 0;      // >= 3: fuse gates into dense blocks of up to this many qubits (standalone sweeps)
@@ -105,6 +106,8 @@ struct Plan {
   unsigned char* d_payload = nullptr;
   aleph_hip_plan_stats stats{};
   bool uploaded = false;
+  // Generated kernel per pass (null: the compiled k_fused_pass), filled by jit_compile_plan.
+  std::vector<hipFunction_t> jit;
   ~Plan();
   void upload(hipStream_t s);
   void execute(hipStream_t s, void* state) const;

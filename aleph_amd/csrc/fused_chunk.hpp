@@ -9,7 +9,7 @@
 
 #include <type_traits>
 
-#include "common.hpp"
+#include "device_types.hpp"
 #include "plan.hpp"
 
 namespace aleph {
@@ -61,14 +61,18 @@ __device__ __forceinline__ uint32_t loff(const uint32_t (&rl)[G], int r) {
   return o;
 }
 
-template <typename T, int G, int B>
+// CTL: kCtlNone / kCtlSome when the plan fixes whether the op has controls
+// (kernels generated per pass), kCtlAny to decide from op->cmask.
+constexpr int kCtlNone = 0, kCtlSome = 1, kCtlAny = 2;
+
+template <typename T, int G, int B, int CTL = kCtlAny>
 __device__ __forceinline__ void op_1q(C2<T> (&a)[1 << G], const DevOp* __restrict__ op, uint64_t tgo,
                                       const uint64_t (&rg)[G]) {
   const T* mm = reinterpret_cast<const T*>(op->m);  // matrix stored in the plan's precision
   const T m00r = mm[0], m00i = mm[1], m01r = mm[2], m01i = mm[3];
   const T m10r = mm[4], m10i = mm[5], m11r = mm[6], m11i = mm[7];
   const uint64_t cm = op->cmask;
-  if (cm == 0) {
+  if (CTL == kCtlNone || (CTL == kCtlAny && cm == 0)) {
 #pragma unroll
     for (int r = 0; r < (1 << G); r++) {
       if (r & (1 << B)) continue;
@@ -147,12 +151,12 @@ __device__ __forceinline__ void quartets_2q(C2<T> (&a)[1 << G], const DevOp* __r
   }
 }
 
-template <typename T, int G, int A, int B>  // A = matrix MSB register bit, B = LSB, A > B
+template <typename T, int G, int A, int B, int CTL = kCtlAny>  // A = matrix MSB register bit, B = LSB, A > B
 __device__ __forceinline__ void op_2q(C2<T> (&a)[1 << G], const DevOp* __restrict__ op, uint64_t tgo,
                                       const uint64_t (&rg)[G]) {
   const uint64_t cm = op->cmask;
   bool on[1 << (G - 2)];
-  if (cm == 0) {
+  if (CTL == kCtlNone || (CTL == kCtlAny && cm == 0)) {
     quartets_2q<T, G, A, B, 0, false>(a, op, on);
     return;
   }
@@ -386,6 +390,79 @@ __device__ __forceinline__ void fused_chunk(C2<T>* __restrict__ state, const Pas
     }
   }
 }
+
+// Stage-by-stage access to one chunk, for pass kernels generated with the
+// pass's op sequence written out (one straight-line body per pass).  The
+// same address maps as fused_chunk: enter(s) selects stage s's thread and
+// register bit maps; load/store move the chunk between HBM and registers;
+// put/get move it between registers and LDS in the current stage's layout.
+template <typename T, int G, bool NT>
+struct ChunkIO {
+  static constexpr int SW = sizeof(T) == 8 ? 3 : 4;
+  static constexpr int N = 1 << G;
+  using V = typename std::conditional<sizeof(T) == 8, v2f64, v2f32>::type;
+  C2<T>* state;
+  const StageDev* stages;
+  C2<T>* lds;
+  uint64_t base;
+  uint64_t tgo;
+  uint64_t rg[G];
+  uint32_t rl[G];
+  uint32_t ts;
+  C2<T>* tp;
+
+  __device__ __forceinline__ ChunkIO(C2<T>* st, const PassDev* pd, const StageDev* sg, C2<T>* l, uint64_t chunk)
+      : state(st), stages(sg), lds(l) {
+    base = chunk;
+    const uint32_t m = pd->m;
+    for (uint32_t i = 0; i < m; i++) base = insert_zero(base, pd->local_pos[i]);
+  }
+  __device__ __forceinline__ void enter(uint32_t s) {
+    const StageDev* sd = stages + s;
+    const uint32_t tid = threadIdx.x;
+    tgo = base;
+    uint32_t tlo = 0;
+    const uint32_t ntb = sd->n_thread_bits;
+    for (uint32_t j = 0; j < ntb; j++) {
+      if ((tid >> j) & 1u) {
+        tgo |= sd->tg[j];
+        tlo |= sd->tl[j];
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      rg[g] = sd->rg[g];
+      rl[g] = sd->rl[g];
+    }
+    tp = state + tgo;
+    ts = swz<SW>(tlo);
+  }
+  __device__ __forceinline__ void load(C2<T> (&a)[N]) const {
+#pragma unroll
+    for (int r = 0; r < N; r++) {
+      const V v = gload<NT>(reinterpret_cast<const V*>(tp + roff<G>(rg, r)));
+      a[r].x = v.x;
+      a[r].y = v.y;
+    }
+  }
+  __device__ __forceinline__ void store(const C2<T> (&a)[N]) const {
+#pragma unroll
+    for (int r = 0; r < N; r++) {
+      V v;
+      v.x = a[r].x;
+      v.y = a[r].y;
+      gstore<NT>(reinterpret_cast<V*>(tp + roff<G>(rg, r)), v);
+    }
+  }
+  __device__ __forceinline__ void put(const C2<T> (&a)[N]) const {
+#pragma unroll
+    for (int r = 0; r < N; r++) lds[ts ^ swz<SW>(loff<G>(rl, r))] = a[r];
+  }
+  __device__ __forceinline__ void get(C2<T> (&a)[N]) const {
+#pragma unroll
+    for (int r = 0; r < N; r++) a[r] = lds[ts ^ swz<SW>(loff<G>(rl, r))];
+  }
+};
 
 }  // namespace fused
 }  // namespace aleph

@@ -14,6 +14,7 @@
 //    item: the waits end.
 // The reference runs such sweeps as one circuit per call
 // (crates/aleph-backend/src/lib.rs:211-248, run_with_outcomes).
+#include "common.hpp"
 #include "fused_chunk.hpp"
 #include "kernels.hpp"
 

@@ -7,6 +7,7 @@
 // and it is written back once: one HBM round trip for the whole pass,
 // instead of one per gate as in the reference's per-gate launches
 // (crates/aleph-cuda/src/sv/backend.rs:515-581).
+#include "common.hpp"
 #include "fused_chunk.hpp"
 #include "kernels.hpp"
 

@@ -96,6 +96,13 @@ class HostPlan:
             self.feature_launch = np.zeros(nrows.value, dtype=np.uint32)
             lib.aleph_hip_plan_features(h, self.features.ctypes.data_as(L.PF32),
                                         self.feature_launch.ctypes.data_as(L.PU32), nrows.value, C.byref(nrows))
+            self.jit_sources = []
+            for i in range(len(self.passes)):
+                n = C.c_size_t()
+                lib.aleph_hip_plan_jit_source(h, i, None, 0, C.byref(n))
+                buf = C.create_string_buffer(n.value + 1)
+                lib.aleph_hip_plan_jit_source(h, i, buf, n.value + 1, C.byref(n))
+                self.jit_sources.append(buf.value.decode())
             st = L.PlanStats()
             lib.aleph_hip_plan_stats_get(h, C.byref(st))
             self.stats = {f: getattr(st, f) for f, _ in L.PlanStats._fields_ if f != "reserved"}

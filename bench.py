@@ -3,7 +3,9 @@
 Default workload (one step = one whole circuit, state reset to |0...0>
 included): the seeded random brickwall — U3(θ,φ,λ) on every qubit each
 layer, CZ on alternating pairs — at n=30 qubits, depth 20, complex128
-amplitudes, planned by the rule-based pass planner.  Also reported on the
+amplitudes, planned by the rule-based pass planner, each fused pass run as a
+kernel generated for that pass and compiled with hiprtc at plan creation
+(`--no-jit`: the compiled op-loop kernel).  Also reported on the
 same line: the FP32 run of the same circuit, the n=30 QFT on a Gaussian
 state, the event-timed roofline of the fused pass kernel, and the CPU
 oracle (a C++ restatement of the reference CPU `sv` backend) timed on a
@@ -57,6 +59,8 @@ def parse():
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 traffic measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched QAOA leg")
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--no-jit", action="store_true",
+                    help="run fused passes with the compiled op-loop kernel instead of per-pass generated kernels")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -66,8 +70,12 @@ def log(msg: str) -> None:
 
 
 def time_circuit(backend, circuit, steps: int, warmup: int, **plan_kw):
-    """Returns (ms per step, plan stats, per-launch ms of the last step, kinds)."""
+    """Returns (ms per step, plan stats, per-launch ms of the last step, kinds,
+    norm, state); plan stats gain `compile_s`, the wall time of plan creation
+    (including kernel generation when asked for)."""
+    t_plan = time.perf_counter()
     plan = backend.compile(circuit, **plan_kw)
+    compile_s = time.perf_counter() - t_plan
     st = backend.allocate(circuit.num_qubits)
     import aleph_amd._lib as L
     lib = L.load()
@@ -85,6 +93,7 @@ def time_circuit(backend, circuit, steps: int, warmup: int, **plan_kw):
     launch_ms, kinds = plan.execute_timed(st)
     norm = backend.norm_sqr(st)
     stats = plan.stats()
+    stats["compile_s"] = compile_s
     plan.free()
     return ms, stats, launch_ms, kinds, norm, st
 
@@ -134,7 +143,8 @@ def cpu_baseline(circuit, budget_s: float):
     }
 
 
-FUSED_KERNEL = "k_fused_pass<double"
+FUSED_KERNEL = "k_fused_pass<double"   # compiled op-loop kernel
+JIT_KERNEL = "aleph_pass"               # per-pass generated kernels
 
 
 def traffic_child(args) -> None:
@@ -145,7 +155,7 @@ def traffic_child(args) -> None:
     from aleph_amd.backend import HipSvBackend
     b = HipSvBackend(seed=1)
     c = W.brickwall(args.n, args.depth, seed=args.seed)
-    plan = b.compile(c, local_qubits=args.local_qubits, max_ops=args.max_ops)
+    plan = b.compile(c, local_qubits=args.local_qubits, max_ops=args.max_ops, jit=not args.no_jit)
     st = b.allocate(args.n)
     plan.execute(st)
     b.norm_sqr(st)
@@ -159,7 +169,7 @@ def _pmc_pass(counter: str, args, out_dir: str) -> dict:
     cmd = ["timeout", "-s", "KILL", "150", "rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", out_dir,
            "-o", counter, "--", sys.executable, os.path.abspath(__file__), "--traffic-child", "--n", str(args.n),
            "--depth", str(args.depth), "--seed", str(args.seed), "--local-qubits", str(args.local_qubits),
-           "--max-ops", str(args.max_ops)]
+           "--max-ops", str(args.max_ops)] + (["--no-jit"] if args.no_jit else [])
     subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd="/tmp")
     sums, counts = {}, {}
     for root, _, files in os.walk(out_dir):
@@ -199,7 +209,8 @@ def measure_traffic(args, state_bytes: float):
     def pick(d, key):
         vals = [v for k, v in d.items() if key in k]
         return float(np.mean(vals)) if vals else None
-    f_fused, w_fused = pick(fetch, FUSED_KERNEL), pick(write, FUSED_KERNEL)
+    fused_name = FUSED_KERNEL if args.no_jit else JIT_KERNEL
+    f_fused, w_fused = pick(fetch, fused_name), pick(write, fused_name)
     f_cal, w_cal = pick(fetch, "k_reduce_branch<double"), pick(write, "k_init_basis<double")
     if None in (f_fused, w_fused, f_cal, w_cal) or f_cal <= 0 or w_cal <= 0:
         return None
@@ -257,7 +268,7 @@ def main():
     n_gates = len(circuit.gates())
     log(f"bench: brickwall n={n} depth={depth} gates={n_gates} fp64")
     b64 = HipSvBackend(seed=1)
-    plan_kw = dict(local_qubits=args.local_qubits, max_ops=args.max_ops)
+    plan_kw = dict(local_qubits=args.local_qubits, max_ops=args.max_ops, jit=not args.no_jit)
     ms, stats, launch_ms, kinds, norm, st = time_circuit(b64, circuit, args.steps, args.warmup, **plan_kw)
     st.free()
     log(f"bench: fp64 {ms:.2f} ms/circuit passes={stats['n_passes']} norm-1={norm - 1:.2e}")
@@ -283,13 +294,16 @@ def main():
         "data": "synthetic (seeded random brickwall, complex128 amplitudes from |0...0>)",
         "config": {"workload": f"random brickwall U3+CZ n={n} depth={depth} seed={args.seed}, FP64, rule planner",
                    "qubits": n, "depth": depth, "gates": n_gates, "passes": stats["n_passes"],
-                   "planner": "rule"},
+                   "planner": "rule",
+                   "codegen": "compiled op loop" if args.no_jit else "per-pass generated kernels (hiprtc)",
+                   "plan_compile_s": stats["compile_s"]},
         "gates_per_s": n_gates / (ms / 1e3),
         "hbm_gbs_circuit": total_bytes / (ms / 1e3) / 1e9,
         "hbm_frac_circuit": total_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_fused_pass<double>", "avg_launch_ms": avg_ms, "launches": int(fused.size),
+                     "kernel": "k_fused_pass<double>" if args.no_jit else "aleph_pass (generated, FP64)",
+                     "avg_launch_ms": avg_ms, "launches": int(fused.size),
                      "algorithmic_bytes_per_launch": sweep_bytes,
                      "fp64": {"algorithmic_flops_per_launch": flops,
                               "achieved_tflops": flops / (avg_ms * 1e-3) / 1e12 if fused.size else 0.0,
@@ -313,7 +327,7 @@ def main():
                 model = CostModel.load(bk)
                 model.install()
                 msl, stl, _, _, norml, s = time_circuit(bk, circuit, args.steps, args.warmup,
-                                                        planner=L.PLANNER_LEARNED)
+                                                        planner=L.PLANNER_LEARNED, jit=not args.no_jit)
                 s.free()
                 log(f"bench: fp{prec} learned {msl:.2f} ms/circuit passes={stl['n_passes']}")
                 out[f"fp{prec}_learned"] = {

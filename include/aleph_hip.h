@@ -185,7 +185,8 @@ typedef struct aleph_hip_plan_opts {
 enum {
   ALEPH_HIP_PLAN_NONTEMPORAL = 1, /* stream chunks with non-temporal loads/stores */
   ALEPH_HIP_PLAN_NO_FUSE_2Q = 2,  /* keep 1q neighbours out of 2q gates (no 4x4 blocks) */
-  ALEPH_HIP_PLAN_NO_FUSE_DIAG = 4 /* keep {diagonal, CNOT} runs gate by gate (no DiagonalPhase fusion) */
+  ALEPH_HIP_PLAN_NO_FUSE_DIAG = 4, /* keep {diagonal, CNOT} runs gate by gate (no DiagonalPhase fusion) */
+  ALEPH_HIP_PLAN_JIT = 8          /* fused passes run as kernels generated per pass (hiprtc at plan creation) */
 };
 
 typedef struct aleph_hip_plan_stats {
@@ -232,6 +233,11 @@ aleph_hip_status aleph_hip_plan_build_host(uint32_t num_qubits, uint32_t precisi
                                            const aleph_hip_plan_opts* opts, aleph_hip_plan** out,
                                            aleph_hip_error* err);
 aleph_hip_status aleph_hip_plan_view_get(const aleph_hip_plan* plan, aleph_hip_plan_view* out);
+/* Diagnostics (no counterpart in the reference): the source of the kernel
+ * ALEPH_HIP_PLAN_JIT generates for fused pass `pass` (works on host-only
+ * plans).  *len receives the length; out (if cap > 0) a NUL-terminated copy. */
+aleph_hip_status aleph_hip_plan_jit_source(const aleph_hip_plan* plan, uint32_t pass, char* out, size_t cap,
+                                           size_t* len);
 /* plan_create + execute + free in one call (run_with_outcomes without measurements, lib.rs:211-248). */
 aleph_hip_status aleph_hip_run_program(aleph_hip_ctx* ctx, aleph_hip_state* st, const uint32_t* words,
                                        size_t n_words, const double* reals, size_t n_reals,
