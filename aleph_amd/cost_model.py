@@ -135,9 +135,10 @@ def record(backend: HipSvBackend, circuit, state, reps: int = 2, **plan_kw):
     return X, y, float(best.sum())
 
 
-def collect(backends, specs, candidates=CANDIDATES, reps: int = 2, log=print):
+def collect(backends, specs, candidates=CANDIDATES, reps: int = 2, log=print, jit: bool = False):
     """Timings of every candidate plan of every circuit; circuit k runs on
-    backends[k % len(backends)] (e.g. alternating FP64 and FP32)."""
+    backends[k % len(backends)] (e.g. alternating FP64 and FP32).  With
+    `jit`, fused passes run as per-pass generated kernels."""
     Xs, ys = [], []
     states = {}
     for k, spec in enumerate(specs):
@@ -147,17 +148,17 @@ def collect(backends, specs, candidates=CANDIDATES, reps: int = 2, log=print):
         if key not in states:
             states[key] = backend.allocate(c.num_qubits)
         for cand in candidates:
-            X, y, _ = record(backend, c, states[key], reps=reps, **cand)
+            X, y, _ = record(backend, c, states[key], reps=reps, jit=jit, **cand)
             Xs.append(X)
             ys.append(y)
-        if log and (k + 1) % 5 == 0:
+        if log:
             log(f"collect: {k + 1}/{len(specs)} circuits, {sum(len(v) for v in ys)} launches")
     for st in states.values():
         st.free()
     return np.concatenate(Xs), np.concatenate(ys)
 
 
-def evaluate(backends, models, specs, reps: int = 2, log=print):
+def evaluate(backends, models, specs, reps: int = 2, log=print, jit: bool = False):
     """Measured ms of the rule plan and of the learned plan per held-out
     circuit (circuit k on backends[k % len(backends)] with models[...])."""
     for m in models:
@@ -170,8 +171,8 @@ def evaluate(backends, models, specs, reps: int = 2, log=print):
         key = (id(backend), c.num_qubits)
         if key not in states:
             states[key] = backend.allocate(c.num_qubits)
-        _, _, rule_ms = record(backend, c, states[key], reps=reps)
-        plan = backend.compile(c, planner=L.PLANNER_LEARNED)
+        _, _, rule_ms = record(backend, c, states[key], reps=reps, jit=jit)
+        plan = backend.compile(c, planner=L.PLANNER_LEARNED, jit=jit)
         st = plan.stats()
         best = None
         for _ in range(reps):
@@ -202,6 +203,7 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--jit", action="store_true", help="time per-pass generated kernels (the default code path)")
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
     backends = [HipSvBackend(seed=1, precision=64), HipSvBackend(seed=1, precision=32)]
@@ -209,7 +211,7 @@ def main(argv=None):
                              d_max=args.d_max)
     train, held = specs[:args.train], specs[args.train:]
     t0 = time.time()
-    X, y = collect(backends, train, log=log)
+    X, y = collect(backends, train, log=log, jit=args.jit)
     log(f"collected {len(y)} launches in {time.time() - t0:.1f} s")
     model = CostModel(backends[0], seed=args.seed)
     t0 = time.time()
@@ -219,12 +221,13 @@ def main(argv=None):
         log(f"train: epoch {e + 10}/{args.epochs}, mse(ln us) {loss:.4f}")
     log(f"trained on the device in {time.time() - t0:.1f} s, mse(ln us) {loss:.4f}")
     twin = CostModel(backends[1], params=model.params())
-    rows = evaluate(backends, [model, twin], held, log=log)
+    rows = evaluate(backends, [model, twin], held, log=log, jit=args.jit)
     gain = float(np.exp(np.mean([math.log(r["rule_ms"] / r["learned_ms"]) for r in rows])))
     model.meta = {"train_circuits": len(train), "launches": int(len(y)), "train_mse": loss,
                   "held_out": rows, "geomean_speedup_vs_rule": gain,
                   "corpus": {"seed": args.seed, "n": [args.n_min, args.n_max], "depth": [20, args.d_max],
-                             "precisions": [64, 32]}}
+                             "precisions": [64, 32]},
+                  "codegen": "generated" if args.jit else "compiled"}
     model.save(args.out)
     print(json.dumps({"launches": int(len(y)), "train_mse": loss, "geomean_speedup_vs_rule": gain}))
 

@@ -17,7 +17,8 @@
 
 namespace aleph {
 
-// Source of the generated kernel `aleph_pass` for fused pass `pass` of `p`.
+// Source of the generated kernel (`aleph_pass_f64` / `aleph_pass_f32`) for
+// fused pass `pass` of `p`.
 std::string jit_pass_source(const Plan& p, uint32_t pass);
 
 // Compile (or take from the process-wide cache) and load a kernel for every

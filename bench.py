@@ -144,7 +144,7 @@ def cpu_baseline(circuit, budget_s: float):
 
 
 FUSED_KERNEL = "k_fused_pass<double"   # compiled op-loop kernel
-JIT_KERNEL = "aleph_pass"               # per-pass generated kernels
+JIT_KERNEL = "aleph_pass_f64"           # per-pass generated kernels
 
 
 def traffic_child(args) -> None:
@@ -302,7 +302,7 @@ def main():
         "hbm_frac_circuit": total_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_fused_pass<double>" if args.no_jit else "aleph_pass (generated, FP64)",
+                     "kernel": "k_fused_pass<double>" if args.no_jit else "aleph_pass_f64 (generated per pass)",
                      "avg_launch_ms": avg_ms, "launches": int(fused.size),
                      "algorithmic_bytes_per_launch": sweep_bytes,
                      "fp64": {"algorithmic_flops_per_launch": flops,

@@ -104,6 +104,7 @@ def test_trained_model_does_not_lose_to_rule_planner():
     b = HipSvBackend(seed=2, precision=64)
     m = CostModel.load(b)
     m.install()
+    jit = m.meta.get("codegen") == "generated"    # time the code path the model was trained on
     specs = W.planner_corpus(8, 999, n_min=22, n_max=24, d_max=80)
     ratios = []
     states = {}
@@ -111,8 +112,8 @@ def test_trained_model_does_not_lose_to_rule_planner():
         c = W.corpus_circuit(spec)
         n = c.num_qubits
         states.setdefault(n, b.allocate(n))
-        _, _, rule_ms = record(b, c, states[n], reps=3)
-        plan = b.compile(c, planner=L.PLANNER_LEARNED)
+        _, _, rule_ms = record(b, c, states[n], reps=3, jit=jit)
+        plan = b.compile(c, planner=L.PLANNER_LEARNED, jit=jit)
         learned_ms = min(plan.execute_timed(states[n])[0].sum() for _ in range(3))
         plan.free()
         ratios.append(rule_ms / learned_ms)

@@ -1,6 +1,7 @@
 """Generated pass kernels on the device: a plan whose fused passes run as
 kernels generated and compiled per pass (ALEPH_HIP_PLAN_JIT) must give the
-same amplitudes as the compiled op-loop kernel for the same plan, and the
+same amplitudes as the compiled op-loop kernel for the same plan (to
+rounding: 1e-14 FP64, 1e-6 FP32), and the
 unfused oracle result (FP64 1e-12, FP32 1e-5), across planner options and
 controlled ops.
 """
@@ -48,7 +49,8 @@ def test_generated_kernels_equal_compiled_kernels(precision, tol, name, make, kw
     b = HipSvBackend(seed=3, precision=precision)
     jit = _run(b, c, psi, jit=True, **kw)
     ref = _run(b, c, psi, **kw)
-    assert max_abs(jit, ref) <= (0.0 if precision == 64 else 1e-6)
+    # Same arithmetic up to the compiler's FMA contraction choices.
+    assert max_abs(jit, ref) <= (1e-14 if precision == 64 else 1e-6)
     o = OracleSv(cap=30)
     st = o.allocate(c.num_qubits)
     st.load(psi)
