@@ -717,6 +717,22 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
       j++;
     }
     sd.n_thread_bits = j;
+    {
+      // Same fold as the device's swz<SW>: bits >= SW XORed into the low SW bits.
+      const uint32_t SW = p.precision == ALEPH_HIP_F64 ? 3u : 4u;
+      for (uint32_t r = 0; r < (1u << pc.G); r++) {
+        uint64_t go = 0;
+        uint32_t lo = 0;
+        for (uint32_t g = 0; g < pc.G; g++)
+          if (r & (1u << g)) {
+            go |= sd.rg[g];
+            lo |= sd.rl[g];
+          }
+        const uint32_t h = lo >> SW;
+        sd.go[r] = go;
+        sd.ls[r] = lo ^ ((h ^ (h >> SW) ^ (h >> (2 * SW)) ^ (h >> (3 * SW))) & ((1u << SW) - 1u));
+      }
+    }
     sd.op_begin = (uint32_t)p.ops.size();
     auto reg_mask = [&](uint64_t phys) {
       uint32_t rm = 0;

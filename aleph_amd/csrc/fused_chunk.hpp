@@ -187,15 +187,18 @@ __device__ __forceinline__ void op_diag(C2<T> (&a)[1 << G], const DevOp* __restr
   }
 }
 
-template <typename T, int G>
+// RM >= 0: the register part of the mask fixed at compile time (generated
+// pass kernels), so only the matching slots are touched; RM < 0: op->a.
+template <typename T, int G, int RM = -1>
 __device__ __forceinline__ void op_sign(C2<T> (&a)[1 << G], const DevOp* __restrict__ op, uint64_t tgo) {
   // Sign-bit XOR with (thread condition as a bit mask) & (uniform slot
   // condition): no divergent branch, one bitop per word.
   const uint64_t mo = op->cmask;
-  const uint32_t rm = op->a;
+  const uint32_t rm = RM >= 0 ? (uint32_t)RM : op->a;
   const uint32_t tm = ((tgo & mo) == mo) ? ~0u : 0u;
 #pragma unroll
   for (int r = 0; r < (1 << G); r++) {
+    if (RM >= 0 && ((uint32_t)r & (uint32_t)RM) != (uint32_t)RM) continue;
     const uint32_t sr = (((uint32_t)r & rm) == rm) ? 0x80000000u : 0u;   // uniform
     a[r].x = flip_sign(a[r].x, sr, tm);
     a[r].y = flip_sign(a[r].y, sr, tm);
@@ -351,22 +354,22 @@ __device__ __forceinline__ void fused_chunk(C2<T>* __restrict__ state, const Pas
       rl[g] = sd->rl[g];
     }
     // Thread and register bits are disjoint, so the global index is the
-    // thread's base plus a uniform register offset, and (swz being linear
-    // over XOR) the LDS slot is swz(thread part) ^ swz(register part), the
-    // latter uniform: one vector op per access instead of the whole fold.
+    // thread's base plus a uniform register offset (sd->go), and (swz being
+    // linear over XOR) the LDS slot is swz(thread part) ^ swz(register part),
+    // the latter precomputed per stage (sd->ls): one vector op per access.
     C2<T>* const tp = state + tgo;
     const uint32_t ts = swz<SW>(tlo);
     if (active) {
       if (s == 0) {
 #pragma unroll
         for (int r = 0; r < N; r++) {
-          const V v = gload<NT>(reinterpret_cast<const V*>(tp + roff<G>(rg, r)));
+          const V v = gload<NT>(reinterpret_cast<const V*>(tp + sd->go[r]));
           a[r].x = v.x;
           a[r].y = v.y;
         }
       } else {
 #pragma unroll
-        for (int r = 0; r < N; r++) a[r] = lds[ts ^ swz<SW>(loff<G>(rl, r))];
+        for (int r = 0; r < N; r++) a[r] = lds[ts ^ sd->ls[r]];
       }
       run_ops<T, G, FEAT>(a, ops, terms, sd->op_begin, sd->op_end, tgo, rg);
     }
@@ -377,14 +380,14 @@ __device__ __forceinline__ void fused_chunk(C2<T>* __restrict__ state, const Pas
           V v;
           v.x = a[r].x;
           v.y = a[r].y;
-          gstore<NT>(reinterpret_cast<V*>(tp + roff<G>(rg, r)), v);
+          gstore<NT>(reinterpret_cast<V*>(tp + sd->go[r]), v);
         }
       }
     } else {
       if (s > 0) __syncthreads();
       if (active) {
 #pragma unroll
-        for (int r = 0; r < N; r++) lds[ts ^ swz<SW>(loff<G>(rl, r))] = a[r];
+        for (int r = 0; r < N; r++) lds[ts ^ sd->ls[r]] = a[r];
       }
       __syncthreads();
     }
@@ -410,6 +413,7 @@ struct ChunkIO {
   uint32_t rl[G];
   uint32_t ts;
   C2<T>* tp;
+  const StageDev* cur;
 
   __device__ __forceinline__ ChunkIO(C2<T>* st, const PassDev* pd, const StageDev* sg, C2<T>* l, uint64_t chunk)
       : state(st), stages(sg), lds(l) {
@@ -419,6 +423,7 @@ struct ChunkIO {
   }
   __device__ __forceinline__ void enter(uint32_t s) {
     const StageDev* sd = stages + s;
+    cur = sd;
     const uint32_t tid = threadIdx.x;
     tgo = base;
     uint32_t tlo = 0;
@@ -440,7 +445,7 @@ struct ChunkIO {
   __device__ __forceinline__ void load(C2<T> (&a)[N]) const {
 #pragma unroll
     for (int r = 0; r < N; r++) {
-      const V v = gload<NT>(reinterpret_cast<const V*>(tp + roff<G>(rg, r)));
+      const V v = gload<NT>(reinterpret_cast<const V*>(tp + cur->go[r]));
       a[r].x = v.x;
       a[r].y = v.y;
     }
@@ -451,16 +456,16 @@ struct ChunkIO {
       V v;
       v.x = a[r].x;
       v.y = a[r].y;
-      gstore<NT>(reinterpret_cast<V*>(tp + roff<G>(rg, r)), v);
+      gstore<NT>(reinterpret_cast<V*>(tp + cur->go[r]), v);
     }
   }
   __device__ __forceinline__ void put(const C2<T> (&a)[N]) const {
 #pragma unroll
-    for (int r = 0; r < N; r++) lds[ts ^ swz<SW>(loff<G>(rl, r))] = a[r];
+    for (int r = 0; r < N; r++) lds[ts ^ cur->ls[r]] = a[r];
   }
   __device__ __forceinline__ void get(C2<T> (&a)[N]) const {
 #pragma unroll
-    for (int r = 0; r < N; r++) a[r] = lds[ts ^ swz<SW>(loff<G>(rl, r))];
+    for (int r = 0; r < N; r++) a[r] = lds[ts ^ cur->ls[r]];
   }
 };
 

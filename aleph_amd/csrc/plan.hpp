@@ -69,6 +69,12 @@ struct alignas(16) StageDev {
   uint64_t rg[kMaxRegBits];      // physical mask of register bit g
   uint32_t rl[kMaxRegBits];      // chunk-local mask of register bit g
   uint32_t pad3;
+  // Uniform parts of the addresses of register slot r, precomputed: the
+  // global offset (OR of rg over the bits of r) and the swizzled LDS slot
+  // of the register part (swz of the OR of rl), so a stage's address maps
+  // cost the kernel one scalar load instead of per-slot arithmetic.
+  uint64_t go[1 << kMaxRegBits];
+  uint32_t ls[1 << kMaxRegBits];
 };
 
 // Op families a pass uses; the launcher picks the kernel variant compiled
