@@ -1,7 +1,9 @@
 """The learned planner's cost model (include/aleph_hip.h, `aleph_hip_model_*`).
 
 A perceptron (16 pass features -> 3 x 64 ReLU -> 1) predicts ln(µs) of
-one fused-pass launch.  Training data are the launch times the engine
+one fused-pass launch, scaled to a 2^24-amplitude state (the planner
+multiplies by 2^(n-24), so the model carries to qubit counts it was not
+trained at).  Training data are the launch times the engine
 records for candidate plans (`aleph_hip_plan_execute_timed`) paired with
 the features of the same launches (`aleph_hip_plan_features`); training,
 inference and the gradient run as the library's own HIP kernels.  With a
@@ -29,6 +31,7 @@ from . import workloads as W
 from .backend import HipSvBackend, _ptr
 
 DEFAULT_PATH = Path(__file__).resolve().parent / "cost_model.json"
+REF_QUBITS = 24   # launch times are learned per 2^24 amplitudes (kCostRefQubits in the library)
 CANDIDATES = [dict(local_qubits=m, max_ops=b, fuse_2q=f) for m in (10, 11, 12) for b in (12, 24, 48)
               for f in (True, False)]
 
@@ -88,7 +91,7 @@ class CostModel:
 
     def save(self, path) -> None:
         Path(path).write_text(json.dumps({"format": "aleph-hip cost model v1", "inputs": L.MODEL_INPUTS,
-                                          "hidden": [64, 64, 64], "target": "ln(microseconds per fused launch)",
+                                          "hidden": [64, 64, 64], "target": "ln(microseconds per fused launch on 2^24 amplitudes)",
                                           "meta": self.meta, "params": [float(v) for v in self.params()]}))
 
     @classmethod
@@ -131,7 +134,9 @@ def record(backend: HipSvBackend, circuit, state, reps: int = 2, **plan_kw):
         ms, _ = plan.execute_timed(state)
         best = ms if best is None else np.minimum(best, ms)
     plan.free()
-    y = np.log(np.maximum(best[idx], 1e-6) * 1e3).astype(np.float32)
+    # Target: ln(µs) scaled to 2^REF_QUBITS amplitudes (the planner scales back by 2^(n - REF_QUBITS)).
+    y = (np.log(np.maximum(best[idx], 1e-6) * 1e3) - (circuit.num_qubits - REF_QUBITS) * math.log(2.0))
+    y = y.astype(np.float32)
     return X, y, float(best.sum())
 
 

@@ -626,7 +626,8 @@ std::unique_ptr<Plan> plan_learned(aleph_hip_ctx* c, const std::vector<ProgInstr
     std::vector<float> pred(R);
     ALEPH_HIP_CHECK(hipMemcpyAsync(pred.data(), dy.p, R * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
-    for (uint32_t r = 0; r < R; r++) cands[owner[r]].us += std::exp((double)std::min(pred[r], 40.f));
+    const double scale = std::ldexp(1.0, (int)n - kCostRefQubits);
+    for (uint32_t r = 0; r < R; r++) cands[owner[r]].us += scale * std::exp((double)std::min(pred[r], 40.f));
   }
   size_t best = 0;
   for (size_t k = 1; k < cands.size(); k++)

@@ -1194,7 +1194,10 @@ void plan_features(const Plan& p, std::vector<float>& rows, std::vector<uint32_t
         if (op.type != OP_SIGN && op.type != OP_PHASE && op.cmask != 0) nctl += 1;
       }
     }
-    const float f[kPlanFeatures] = {(float)p.n / 32.f,
+    // Slot 0 stays 0: the model predicts a launch on 2^kCostRefQubits
+    // amplitudes and the planner scales by the state size, so predictions
+    // hold at qubit counts outside the training corpus.
+    const float f[kPlanFeatures] = {0.f,
                                     p.precision == ALEPH_HIP_F64 ? 1.f : 0.f,
                                     (float)pd.m / 13.f,
                                     (float)pd.n_stages / 8.f,

@@ -121,6 +121,11 @@ std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog, uint32_t n,
 // Greedy fusion of gates into dense blocks of at most `width` qubits.
 std::vector<ProgInstr> fuse_dense_blocks(const std::vector<ProgInstr>& prog, uint32_t width);
 
+// The cost model predicts ln(µs) of a fused launch scaled to a state of
+// 2^kCostRefQubits amplitudes (a pass streams the whole state, so its time
+// is proportional to 2^n at the sizes that matter); the planner multiplies
+// by 2^(n - kCostRefQubits).
+constexpr int kCostRefQubits = 24;
 // Cost-model features of every fused-pass launch of `p` (kPlanFeatures
 // floats per row, launch order); `launch` receives each row's step index.
 constexpr int kPlanFeatures = 16;

@@ -264,8 +264,9 @@ aleph_hip_status aleph_hip_probabilities(aleph_hip_ctx* ctx, aleph_hip_state* st
 
 /* ---- the learned planner's cost model ----
  * A perceptron (ALEPH_HIP_MODEL_INPUTS features -> 3 x 64 ReLU -> 1, FP32)
- * predicting ln(microseconds) of one fused-pass launch from the pass's
- * features (aleph_hip_plan_features).  It is trained on the device by the
+ * predicting ln(microseconds) of one fused-pass launch on a 2^24-amplitude
+ * state from the pass's features (aleph_hip_plan_features; slot 0 is unused
+ * and 0); the planner scales each prediction by 2^(n - 24).  It is trained on the device by the
  * library's own forward, gradient and Adam kernels, on launch times the
  * engine records (aleph_hip_plan_execute_timed).  With a model set on the
  * context, ALEPH_HIP_PLANNER_LEARNED plans every candidate (chunk qubits

@@ -60,7 +60,7 @@ def test_plan_features_describe_the_plan():
     assert hp.features.shape == (len(fused), L.MODEL_INPUTS)
     n2q = sum(1 for op in hp.ops if op.type == 2)
     assert round(float(hp.features[:, 5].sum() * 32)) == n2q
-    assert np.allclose(hp.features[:, 0], 20 / 32) and np.allclose(hp.features[:, 1], 1.0)
+    assert np.all(hp.features[:, 0] == 0.0) and np.allclose(hp.features[:, 1], 1.0)   # slot 0 unused
     q = HostPlan(W.qft(12), precision=32)
     assert np.all(q.features[:, 1] == 0.0) and q.features[:, 12].max() == 1.0
 

@@ -37,6 +37,7 @@ CASES = [
     ("random_m10", lambda rng: random_circuit(13, 120, rng, kq=False), {"local_qubits": 10}),
     ("random_g5", lambda rng: random_circuit(14, 120, rng, kq=False), {"reg_bits": 5, "local_qubits": 12}),
     ("brickwall_nt", lambda rng: W.brickwall(17, 8, seed=9), {"nontemporal": True, "fuse_2q": False}),
+    ("hea_m10", lambda rng: W.hea(15, 4, seed=6), {"local_qubits": 10}),
 ]
 
 
