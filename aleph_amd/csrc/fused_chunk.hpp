@@ -234,6 +234,54 @@ __device__ __forceinline__ void op_phase(C2<T> (&a)[1 << G], const DevOp* __rest
     acc[k] = sum_terms(terms, op->term_off + bounds[k], op->term_off + bounds[k + 1], tgo);
   const uint32_t end = op->term_off + op->n_terms;
   const uint32_t gen_end = end + op->b;
+  if (op->b == 0) {
+    // No terms on three or more register bits: e^{i phi_r} is a product of
+    // the class exponentials.  One sincos per class (1 + G + G(G-1)/2)
+    // instead of one per amplitude; slot r's factor is built from slot
+    // r without its top bit times e^{i(linear(top) + sum pair(g, top))}
+    // over the remaining bits g, i.e. popcount(r) complex products.
+    // Classes without terms (the common case: a controlled-phase ladder
+    // fills 4 or 5 of the 11) contribute the factor 1: no sincos.
+    double ec[L::kCount], es[L::kCount];
+#pragma unroll
+    for (int k = 0; k < L::kCount; k++) {
+      if (bounds[k] == bounds[k + 1]) {
+        ec[k] = 1.0;
+        es[k] = 0.0;
+      } else {
+        sincos(acc[k], &es[k], &ec[k]);
+      }
+    }
+    double fr[1 << G], fi[1 << G];
+    fr[0] = ec[0];
+    fi[0] = es[0];
+#pragma unroll
+    for (int r = 1; r < (1 << G); r++) {
+      int top = 0;
+#pragma unroll
+      for (int g = 0; g < G; g++)
+        if (r & (1 << g)) top = g;
+      const int rest = r & ~(1 << top);
+      double cr = ec[L::linear(top)], ci = es[L::linear(top)];
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        if (!(rest & (1 << g))) continue;
+        const double pr = ec[L::pair(g, top)], pi = es[L::pair(g, top)];
+        const double t = cr * pr - ci * pi;
+        ci = cr * pi + ci * pr;
+        cr = t;
+      }
+      fr[r] = fr[rest] * cr - fi[rest] * ci;
+      fi[r] = fr[rest] * ci + fi[rest] * cr;
+    }
+#pragma unroll
+    for (int r = 0; r < (1 << G); r++) {
+      const C2<T> x = a[r];
+      a[r].x = x.x * (T)fr[r] - x.y * (T)fi[r];
+      a[r].y = x.x * (T)fi[r] + x.y * (T)fr[r];
+    }
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < (1 << G); r++) {
     double phi = acc[0];
