@@ -515,6 +515,26 @@ struct ChunkIO {
 #pragma unroll
     for (int r = 0; r < N; r++) a[r] = lds[ts ^ cur->ls[r]];
   }
+  // Stage change through a buffer of 2^m scalars (half the LDS of put/get):
+  // real parts first, then imaginary parts, each written in the current
+  // stage's layout and read in stage `next`'s.  Ends in stage `next`.
+  // The caller orders earlier reads of `buf` before this call.
+  __device__ __forceinline__ void exchange_split(C2<T> (&a)[N], uint32_t next, T* __restrict__ buf) {
+    const uint32_t ots = ts;
+    const StageDev* ocur = cur;
+#pragma unroll
+    for (int r = 0; r < N; r++) buf[ots ^ ocur->ls[r]] = a[r].x;
+    __syncthreads();
+    enter(next);
+#pragma unroll
+    for (int r = 0; r < N; r++) a[r].x = buf[ts ^ cur->ls[r]];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < N; r++) buf[ots ^ ocur->ls[r]] = a[r].y;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < N; r++) a[r].y = buf[ts ^ cur->ls[r]];
+  }
 };
 
 }  // namespace fused

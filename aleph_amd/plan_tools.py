@@ -110,6 +110,7 @@ class HostPlan:
         finally:
             lib.aleph_hip_plan_free(h)
         self.num_qubits = circuit.num_qubits
+        self.precision = precision
 
     def summary(self) -> list[dict]:
         out = []

@@ -56,10 +56,17 @@ def test_sources_list_the_pass_ops_in_order():
             want = [OP_CALL[hp.ops[o].type] for sd in stages for o in range(sd.op_begin, sd.op_end)]
             got = re.findall(r"(op_\w+<)", src)
             assert got == want
-            assert src.count("io.enter(") == p.n_stages
             assert src.count("io.load(a)") == 1 and src.count("io.store(a)") == 1
-            assert src.count("io.put(a)") == p.n_stages - 1 == src.count("io.get(a)")
-            assert f"__launch_bounds__({1 << (p.m - p.reg_bits)}, 2)" in src
+            if hp.precision == 64 and p.n_stages > 1:
+                # FP64: real and imaginary parts exchanged in turn through half the LDS
+                assert src.count("io.exchange_split(a, ") == p.n_stages - 1
+                assert src.count("io.enter(") == 1 and "io.put(" not in src
+                assert f"__launch_bounds__({1 << (p.m - p.reg_bits)}, 4)" in src
+            else:
+                assert src.count("io.enter(") == p.n_stages
+                assert src.count("io.put(a)") == p.n_stages - 1 == src.count("io.get(a)")
+                assert f"__launch_bounds__({1 << (p.m - p.reg_bits)}, 2)" in src
+
 
 
 @pytest.mark.skipif(not HIPRTC.exists(), reason="hiprtc not installed")
