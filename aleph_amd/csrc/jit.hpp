@@ -22,9 +22,9 @@ namespace aleph {
 std::string jit_pass_source(const Plan& p, uint32_t pass);
 
 // Compile (or take from the process-wide cache) and load a kernel for every
-// fused pass of `p` on the current device.  Passes with phase-polynomial ops
-// and passes whose chunk does not fit 64 KiB of static LDS keep the compiled
-// kernel.
+// fused pass of `p` on the current device.  Passes with many phase-polynomial
+// ops and passes whose chunk does not fit 64 KiB of static LDS keep the
+// compiled kernel.
 void jit_compile_plan(Plan& p);
 
 // Launch step `i` of `p` (a fused pass that has a generated kernel).
