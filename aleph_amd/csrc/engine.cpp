@@ -731,6 +731,7 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
         const uint32_t h = lo >> SW;
         sd.go[r] = go;
         sd.ls[r] = lo ^ ((h ^ (h >> SW) ^ (h >> (2 * SW)) ^ (h >> (3 * SW))) & ((1u << SW) - 1u));
+        sd.lb[r] = sd.ls[r] * 8u;
       }
     }
     sd.op_begin = (uint32_t)p.ops.size();

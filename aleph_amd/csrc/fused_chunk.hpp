@@ -519,21 +519,27 @@ struct ChunkIO {
   // real parts first, then imaginary parts, each written in the current
   // stage's layout and read in stage `next`'s.  Ends in stage `next`.
   // The caller orders earlier reads of `buf` before this call.
+  // Addresses are formed in bytes ((t ^ l) * 8 == t * 8 ^ l * 8) with the
+  // register part's byte offsets precomputed (StageDev::lb): one XOR per
+  // access.
   __device__ __forceinline__ void exchange_split(C2<T> (&a)[N], uint32_t next, T* __restrict__ buf) {
-    const uint32_t ots = ts;
+    static_assert(sizeof(T) == 8, "StageDev::lb holds offsets of 8-byte elements");
+    char* const b = reinterpret_cast<char*>(buf);
+    const uint32_t ob = ts * (uint32_t)sizeof(T);
     const StageDev* ocur = cur;
 #pragma unroll
-    for (int r = 0; r < N; r++) buf[ots ^ ocur->ls[r]] = a[r].x;
+    for (int r = 0; r < N; r++) *reinterpret_cast<T*>(b + (ob ^ ocur->lb[r])) = a[r].x;
     __syncthreads();
     enter(next);
+    const uint32_t nb = ts * (uint32_t)sizeof(T);
 #pragma unroll
-    for (int r = 0; r < N; r++) a[r].x = buf[ts ^ cur->ls[r]];
+    for (int r = 0; r < N; r++) a[r].x = *reinterpret_cast<const T*>(b + (nb ^ cur->lb[r]));
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < N; r++) buf[ots ^ ocur->ls[r]] = a[r].y;
+    for (int r = 0; r < N; r++) *reinterpret_cast<T*>(b + (ob ^ ocur->lb[r])) = a[r].y;
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < N; r++) a[r].y = buf[ts ^ cur->ls[r]];
+    for (int r = 0; r < N; r++) a[r].y = *reinterpret_cast<const T*>(b + (nb ^ cur->lb[r]));
   }
 };
 

@@ -75,6 +75,7 @@ struct alignas(16) StageDev {
   // cost the kernel one scalar load instead of per-slot arithmetic.
   uint64_t go[1 << kMaxRegBits];
   uint32_t ls[1 << kMaxRegBits];
+  uint32_t lb[1 << kMaxRegBits];   // ls[r] * 8: byte offset in an LDS array of 8-byte scalars
 };
 
 // Op families a pass uses; the launcher picks the kernel variant compiled

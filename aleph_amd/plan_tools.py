@@ -23,7 +23,7 @@ class PassDev(C.Structure):
 class StageDev(C.Structure):
     _fields_ = [("op_begin", u32), ("op_end", u32), ("n_thread_bits", u32), ("pad", u32), ("tg", u64 * 10),
                 ("tl", u32 * 10), ("pad2", u32 * 2), ("rg", u64 * 5), ("rl", u32 * 5), ("pad3", u32),
-                ("go", u64 * 32), ("ls", u32 * 32)]
+                ("go", u64 * 32), ("ls", u32 * 32), ("lb", u32 * 32)]
 
 
 class DevOp(C.Structure):
