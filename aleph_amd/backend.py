@@ -280,6 +280,21 @@ class HipSvBackend:
                                                    C.byref(out)))
         return float(out.value)
 
+    def expectation_pauli_sum(self, state: HipState, paulis) -> float:
+        """sum of <P> over PauliStrings (each with its coefficient), one call
+        (Backend::expectation_pauli_sum, crates/aleph-backend/src/lib.rs:422-432)."""
+        paulis = list(paulis)
+        coeffs = np.asarray([float(p.coefficient) for p in paulis] or [0.0], dtype=np.float64)
+        offs = np.zeros(len(paulis) + 1, dtype=np.uint32)
+        offs[1:] = np.cumsum([len(p.terms) for p in paulis]) if paulis else []
+        qs = _u32([q for p in paulis for q, _ in p.terms] or [0])
+        ps = np.asarray([PAULI_CODE[c] for p in paulis for _, c in p.terms] or [0], dtype=np.uint8)
+        out = C.c_double()
+        self._check(L.load().aleph_hip_expectation_sum(self._ctx, state.handle, len(paulis), _ptr(coeffs, C.c_double),
+                                                       _ptr(offs, C.c_uint32), _ptr(qs, C.c_uint32),
+                                                       _ptr(ps, C.c_uint8), C.byref(out)))
+        return float(out.value)
+
     def probabilities(self, state: HipState, qubits) -> np.ndarray:
         qs = _u32(list(qubits) or [0])
         out = np.zeros(1 << len(qubits), dtype=np.float64)

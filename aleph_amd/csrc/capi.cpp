@@ -172,7 +172,8 @@ void validate(aleph_hip_ctx* c, aleph_hip_state* st, double* p0, double* p1, int
   double r[3];
   reduce(c, st, q, r);
   const double total = r[0] + r[1];
-  const double tol = std::sqrt(std::ldexp(1.0, (int)st->n)) * kAmplitudeTol;
+  const double tol = st->precision == ALEPH_HIP_F32 ? kFp32NormDrift
+                                                    : std::sqrt(std::ldexp(1.0, (int)st->n)) * kAmplitudeTol;
   if (r[2] != 0.0) throw make_failure(ALEPH_HIP_ERR_INVALID_STATE, "non-finite amplitude norm²", "invalid state");
   if (!std::isfinite(total) || std::fabs(total - 1.0) > tol)
     throw make_failure(ALEPH_HIP_ERR_INVALID_STATE, "state norm² deviates from 1 beyond drift budget",
@@ -871,6 +872,70 @@ aleph_hip_status aleph_hip_expectation(aleph_hip_ctx* c, aleph_hip_state* st, do
       default: v = im; break;
     }
     *out = coeff * v;
+  });
+}
+
+aleph_hip_status aleph_hip_expectation_sum(aleph_hip_ctx* c, aleph_hip_state* st, uint32_t n_terms,
+                                           const double* coeffs, const uint32_t* offsets, const uint32_t* qubits,
+                                           const uint8_t* paulis, double* out) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(out != nullptr && (n_terms == 0 || (coeffs && offsets)), "null argument");
+    validate(c, st, nullptr, nullptr, -1);
+    struct Term {
+      uint64_t x, zy;
+      int ny;
+      double coeff;
+    };
+    std::vector<Term> terms(n_terms);
+    for (uint32_t t = 0; t < n_terms; t++) {
+      need(offsets[t] <= offsets[t + 1], "term offsets must be non-decreasing");
+      const uint32_t nt = offsets[t + 1] - offsets[t];
+      need(nt == 0 || (qubits && paulis), "null argument");
+      if (!std::isfinite(coeffs[t]))
+        throw make_failure(ALEPH_HIP_ERR_INVALID_PAULI_STRING, "non-finite coefficient", "invalid Pauli string");
+      check_distinct(qubits + offsets[t], nt, st, "expectation");
+      Term& tm = terms[t];
+      tm = Term{0, 0, 0, coeffs[t]};
+      for (uint32_t i = offsets[t]; i < offsets[t + 1]; i++) {
+        need(paulis[i] <= 3, "pauli code must be 0..3");
+        const uint64_t b = 1ull << qubits[i];
+        if (paulis[i] == 1) tm.x |= b;
+        if (paulis[i] == 2) { tm.x |= b; tm.zy |= b; tm.ny++; }
+        if (paulis[i] == 3) tm.zy |= b;
+      }
+    }
+    // Each term is one read of the state into its own (re, im) pair in the
+    // context scratch; terms are batched so the pairs fit after the partials.
+    double* partials = c->d_scratch;
+    double* res = c->d_scratch + 3 * kReduceBlocks;
+    const uint32_t batch = 1024;
+    std::vector<double> host(2 * batch);
+    double total = 0.0;
+    for (uint32_t t0 = 0; t0 < n_terms; t0 += batch) {
+      const uint32_t nb = std::min(batch, n_terms - t0);
+      for (uint32_t j = 0; j < nb; j++) {
+        const Term& tm = terms[t0 + j];
+        if (st->precision == ALEPH_HIP_F64)
+          launch_expect_pauli<double>(c->stream, st->d, st->n, tm.x, tm.zy, partials, res + 2 * j);
+        else
+          launch_expect_pauli<float>(c->stream, st->d, st->n, tm.x, tm.zy, partials, res + 2 * j);
+      }
+      ALEPH_HIP_CHECK(hipMemcpyAsync(host.data(), res, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+      for (uint32_t j = 0; j < nb; j++) {
+        const double re = host[2 * j], im = host[2 * j + 1];
+        double v;
+        switch (terms[t0 + j].ny & 3) {  // Re(i^ny * (re + i im))
+          case 0: v = re; break;
+          case 1: v = -im; break;
+          case 2: v = -re; break;
+          default: v = im; break;
+        }
+        total += terms[t0 + j].coeff * v;
+      }
+    }
+    *out = total;
   });
 }
 

@@ -19,6 +19,10 @@ namespace aleph {
 using cplx = std::complex<double>;
 
 constexpr double kAmplitudeTol = 1e-10;    // crates/aleph-core/src/lib.rs:53
+// Norm² drift budget of FP32 states: a fixed 1e-3, as the reference's GPU
+// FP32 readout (crates/aleph-cuda/src/sv/readout_f32.rs:34-39); the FP64
+// budget sqrt(2^n) * kAmplitudeTol would reject every legitimate FP32 state.
+constexpr double kFp32NormDrift = 1e-3;
 constexpr double kDiagonalEpsSq = 1e-30;   // crates/aleph-sv/src/kernels/mod.rs:295
 constexpr uint32_t kMaxFusedK = 6;         // widest dense block this backend applies
 

@@ -258,6 +258,13 @@ aleph_hip_status aleph_hip_sample_with_uniforms(aleph_hip_ctx* ctx, aleph_hip_st
 aleph_hip_status aleph_hip_expectation(aleph_hip_ctx* ctx, aleph_hip_state* st, double coefficient,
                                        const uint32_t* qubits, const uint8_t* paulis, uint32_t n_terms,
                                        double* out);
+/* sum_t coeffs[t] * <psi|P_t|psi> for n_terms Pauli strings; term t acts on
+ * qubits[offsets[t] .. offsets[t+1]) with paulis[...] (codes as above).
+ * Replaces Backend::expectation_pauli_sum (crates/aleph-backend/src/lib.rs:422-432):
+ * one validation, one read of the state per term, one host synchronisation. */
+aleph_hip_status aleph_hip_expectation_sum(aleph_hip_ctx* ctx, aleph_hip_state* st, uint32_t n_terms,
+                                           const double* coeffs, const uint32_t* offsets, const uint32_t* qubits,
+                                           const uint8_t* paulis, double* out);
 /* Marginal over `qubits` (qubits[0] = LSB of the output index), 2^n_qubits doubles. */
 aleph_hip_status aleph_hip_probabilities(aleph_hip_ctx* ctx, aleph_hip_state* st, const uint32_t* qubits,
                                          uint32_t n_qubits, double* out);

@@ -242,6 +242,45 @@ def test_norm_measure_expectation_probabilities(hip, orc):
     st.free()
 
 
+@pytest.mark.parametrize("precision,tol", [(64, 1e-12), (32, 1e-5)])
+def test_expectation_pauli_sum(orc, precision, tol):
+    """Backend::expectation_pauli_sum (crates/aleph-backend/src/lib.rs:422-432):
+    one call over many Pauli strings equals the sum of the oracle's
+    per-string expectations; a MaxCut Hamiltonian sum_edges (1 - Z_u Z_v)/2
+    (maxcut_pauli_sum, crates/aleph-ir/src/ansatz.rs) equals sum_x |psi_x|^2 cut(x)."""
+    n = 11
+    hip = B.HipSvBackend(seed=4, precision=precision)
+    c = W.brickwall(n, 6, seed=5)
+    st = hip.allocate(n)
+    hip.run_program(st, c)
+    o = orc.OracleSv()
+    ost = o.run(c)
+    rng = np.random.default_rng(21)
+    terms = []
+    for _ in range(40):
+        k = int(rng.integers(0, 6))
+        qs = rng.choice(n, k, replace=False)
+        terms.append(PauliString.new(float(rng.uniform(-2, 2)), [(int(q), "XYZ"[rng.integers(3)]) for q in qs]))
+    want = sum(o.expectation_value(ost, p) for p in terms)
+    assert abs(hip.expectation_pauli_sum(st, terms) - want) < tol * len(terms)
+    assert hip.expectation_pauli_sum(st, []) == 0.0
+    edges = W.random_3_regular(10, 7)
+    ham = [PauliString.new(0.5, []) for _ in edges] + [PauliString.new(-0.5, [(u, "Z"), (v, "Z")]) for u, v in edges]
+    probs = np.abs(ost.amplitudes()) ** 2
+    idx = np.arange(1 << n)
+    cut = sum(((idx >> u) ^ (idx >> v)) & 1 for u, v in edges)
+    assert abs(hip.expectation_pauli_sum(st, ham) - float(np.sum(probs * cut))) < tol * len(ham)
+    with pytest.raises(B.QubitOutOfRange):
+        hip.expectation_pauli_sum(st, [PauliString.new(1.0, [(n + 3, "Z")])])
+    # The drift budget: FP64 sqrt(2^n) * 1e-10, FP32 a fixed 1e-3
+    # (crates/aleph-cuda/src/sv/readout_f32.rs:34-39); a state scaled by 0.99
+    # (norm² 0.98) is rejected in both precisions.
+    st.load(0.99 * ost.amplitudes())
+    with pytest.raises(B.InvalidState):
+        hip.expectation_pauli_sum(st, terms)
+    st.free()
+
+
 def test_readout_errors(hip):
     st = hip.allocate(2)
     with pytest.raises(B.QubitOutOfRange):
