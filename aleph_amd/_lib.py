@@ -149,6 +149,7 @@ SIGNATURES = {
     "aleph_hip_sample_with_uniforms": (i32, [P, P, PF64, u32, PU64]),
     "aleph_hip_expectation": (i32, [P, P, f64, PU32, PU8, u32, PF64]),
     "aleph_hip_expectation_sum": (i32, [P, P, u32, PF64, PU32, PU32, PU8, PF64]),
+    "aleph_hip_linear_xeb": (i32, [P, P, PF64]),
     "aleph_hip_probabilities": (i32, [P, P, PU32, u32, PF64]),
     "aleph_hip_batch_create": (i32, [P, u32, PU32, u32, PU32, PU64, PF64, PU64, C.POINTER(BatchOpts),
                                      C.POINTER(P)]),

@@ -322,6 +322,12 @@ class HipSvBackend:
             raise InvalidArgument("permutation length must equal num_qubits")
         self._check(L.load().aleph_hip_unpermute(self._ctx, state.handle, _ptr(p, C.c_uint32)))
 
+    def linear_xeb(self, state: HipState) -> float:
+        """2^n * sum_x p(x)^2 - 1 of the exact state (benches/src/lib.rs:29 linear_xeb)."""
+        out = C.c_double()
+        self._check(L.load().aleph_hip_linear_xeb(self._ctx, state.handle, C.byref(out)))
+        return float(out.value)
+
     def norm_sqr(self, state: HipState) -> float:
         out = C.c_double()
         self._check(L.load().aleph_hip_norm_sqr(self._ctx, state.handle, C.byref(out)))

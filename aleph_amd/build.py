@@ -12,6 +12,7 @@ import concurrent.futures as cf
 import os
 import subprocess
 import sys
+import time
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -37,15 +38,26 @@ COMMON = [
 ]
 
 
-def _compile(src: str) -> tuple[str, str]:
+def _newest_input() -> float:
+    """Newest modification time among the headers every source may include."""
+    paths = [*CSRC.glob("*.hpp"), *(PKG.parent / "include").glob("*.h"), BUILD / "jit_headers.inc", Path(__file__)]
+    return max(p.stat().st_mtime for p in paths if p.exists())
+
+
+def _compile(src: str, incremental: bool = False, headers_mtime: float = 0.0) -> tuple[str, str, float]:
     out = BUILD / (src + ".o")
+    if incremental and out.exists():
+        built = out.stat().st_mtime
+        if built > (CSRC / src).stat().st_mtime and built > headers_mtime:
+            return str(out), "", 0.0
     cmd = [HIPCC, *COMMON, "-I", str(PKG.parent / "include"), "-I", str(BUILD), "-c", str(CSRC / src), "-o", str(out)]
     if src.endswith(".cpp"):
         cmd[1:1] = ["-x", "hip"]
+    t0 = time.time()
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{res.stdout}\n{res.stderr}")
-    return str(out), res.stderr
+    return str(out), res.stderr, time.time() - t0
 
 
 # Device headers the generated pass kernels compile against (hiprtc sees
@@ -70,16 +82,20 @@ def _write_jit_headers() -> None:
         out.write_text(new)
 
 
-def build(verbose: bool = False) -> Path:
+def build(verbose: bool = False, incremental: bool = False) -> Path:
+    """Compile every source (or, with `incremental`, only those older than
+    their inputs) and link the library."""
     BUILD.mkdir(exist_ok=True)
     _write_jit_headers()
+    hdr = _newest_input()
     with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
-        results = list(ex.map(_compile, SOURCES))
+        results = list(ex.map(lambda s: _compile(s, incremental, hdr), SOURCES))
     objs = [r[0] for r in results]
     if verbose:
-        for _, err in results:
+        for src, (_, err, secs) in zip(SOURCES, results):
             if err.strip():
                 print(err, file=sys.stderr)
+            print(f"{src}: {secs:.1f} s", file=sys.stderr)
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs, "-lhiprtc"]
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -90,4 +106,4 @@ def build(verbose: bool = False) -> Path:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, incremental="--incremental" in sys.argv))

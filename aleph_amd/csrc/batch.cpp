@@ -303,7 +303,22 @@ void record_batch(Batch& b, hipStream_t s) {
 }
 }  // namespace
 
+Batch::GraphKey Batch::current_key() const {
+  GraphKey k;
+  k.mode = cfg.mode;
+  k.grid = grid;
+  k.threads = launch.threads;
+  k.n_items = (uint32_t)items.size();
+  k.lds_bytes = launch.lds_bytes;
+  k.nontemporal = launch.nontemporal;
+  k.round_begin = round_begin;
+  k.ptrs = {pool, d_passes, d_stages, d_ops, d_terms, d_items, d_offs, d_done, d_ctl};
+  return k;
+}
+
 void Batch::run(hipStream_t s) {
+  GraphKey key = current_key();
+  if (graph && !(key == graph_key)) drop_graph();
   if (!graph) {
     ALEPH_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
@@ -317,6 +332,7 @@ void Batch::run(hipStream_t s) {
     }
     ALEPH_HIP_CHECK(hipStreamEndCapture(s, &graph_src));
     ALEPH_HIP_CHECK(hipGraphInstantiate(&graph, graph_src, nullptr, nullptr, 0));
+    graph_key = std::move(key);
   }
   ALEPH_HIP_CHECK(hipGraphLaunch(graph, s));
 }

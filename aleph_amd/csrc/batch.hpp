@@ -74,6 +74,23 @@ struct Batch {
   size_t cap_passes = 0, cap_stages = 0, cap_ops = 0, cap_terms = 0, cap_items = 0;
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_src = nullptr;
+  // What the captured graph baked in: launch geometry, per-round item
+  // ranges and device pointers.  A replay with any of them changed would
+  // run the old launches over the new plans, so run() re-captures instead.
+  struct GraphKey {
+    uint32_t mode = 0, grid = 0, threads = 0, n_items = 0;
+    size_t lds_bytes = 0;
+    bool nontemporal = false;
+    std::vector<uint32_t> round_begin;
+    std::vector<const void*> ptrs;
+    bool operator==(const GraphKey& o) const {
+      return mode == o.mode && grid == o.grid && threads == o.threads && n_items == o.n_items &&
+             lds_bytes == o.lds_bytes && nontemporal == o.nontemporal && round_begin == o.round_begin &&
+             ptrs == o.ptrs;
+    }
+  };
+  GraphKey graph_key;
+  GraphKey current_key() const;
 
   ~Batch();
   // Plan every instance with `cfg` and pack the plans (host only).

@@ -60,7 +60,11 @@ struct aleph_hip_model {
 
 namespace {
 
-constexpr size_t kScratchBytes = 1u << 20;
+constexpr size_t kScratchBytes = 2u << 20;
+// Context scratch layout (doubles): reduction partials from 0 (at most
+// kExpectGroup * kReduceBlocks of them), results from kResOffset.
+constexpr size_t kResOffset = (size_t)kExpectGroup * kReduceBlocks;
+static_assert((kResOffset + 4096) * sizeof(double) <= kScratchBytes, "scratch layout");
 
 // xoshiro256** seeded through SplitMix64.
 void rng_seed(aleph_hip_ctx* c, uint64_t seed) {
@@ -159,7 +163,7 @@ size_t amp_bytes(const aleph_hip_state* st) { return st->precision == ALEPH_HIP_
 // Σ|ψ|² over bit q clear / set, non-finite count (q < 0: everything in [0]).
 void reduce(aleph_hip_ctx* c, aleph_hip_state* st, int32_t q, double out[3]) {
   double* partials = c->d_scratch;
-  double* res = c->d_scratch + 3 * kReduceBlocks;
+  double* res = c->d_scratch + kResOffset;
   if (st->precision == ALEPH_HIP_F64) launch_reduce_branch<double>(c->stream, st->d, st->n, q, partials, res);
   else launch_reduce_branch<float>(c->stream, st->d, st->n, q, partials, res);
   ALEPH_HIP_CHECK(hipMemcpyAsync(c->h_pinned, res, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -391,6 +395,9 @@ aleph_hip_status aleph_hip_state_load(aleph_hip_ctx* c, aleph_hip_state* st, con
   return guarded(c, [&] {
     check_state(c, st);
     need(amps != nullptr, "null amplitudes");
+    // Launches on the context stream (a non-blocking stream) may still be
+    // writing the state: finish them before the copy overwrites it.
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
     const size_t len = 1ull << st->n;
     if (st->precision == ALEPH_HIP_F64) {
       ALEPH_HIP_CHECK(hipMemcpy(st->d, amps, len * 16, hipMemcpyHostToDevice));
@@ -858,7 +865,7 @@ aleph_hip_status aleph_hip_expectation(aleph_hip_ctx* c, aleph_hip_state* st, do
       if (paulis[i] == 3) zy |= b;
     }
     double* partials = c->d_scratch;
-    double* res = c->d_scratch + 3 * kReduceBlocks;
+    double* res = c->d_scratch + kResOffset;
     if (st->precision == ALEPH_HIP_F64) launch_expect_pauli<double>(c->stream, st->d, st->n, x, zy, partials, res);
     else launch_expect_pauli<float>(c->stream, st->d, st->n, x, zy, partials, res);
     ALEPH_HIP_CHECK(hipMemcpyAsync(c->h_pinned, res, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -905,37 +912,90 @@ aleph_hip_status aleph_hip_expectation_sum(aleph_hip_ctx* c, aleph_hip_state* st
         if (paulis[i] == 3) tm.zy |= b;
       }
     }
-    // Each term is one read of the state into its own (re, im) pair in the
-    // context scratch; terms are batched so the pairs fit after the partials.
+    // Terms are grouped by X mask; each group of up to kExpectGroup terms is
+    // one read of the state (pairs {i, i ^ x} read once each), so a
+    // diagonal Hamiltonian such as a MaxCut sum costs one sweep per 16 terms.
     double* partials = c->d_scratch;
-    double* res = c->d_scratch + 3 * kReduceBlocks;
-    const uint32_t batch = 1024;
-    std::vector<double> host(2 * batch);
+    double* res = c->d_scratch + kResOffset;
+    std::vector<uint32_t> order(n_terms);
+    for (uint32_t t = 0; t < n_terms; t++) order[t] = t;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return terms[a].x < terms[b].x; });
+    struct Launch {
+      uint64_t x;
+      std::vector<uint32_t> ids;
+    };
+    std::vector<Launch> launches;
+    for (uint32_t k = 0; k < n_terms; k++) {
+      const Term& tm = terms[order[k]];
+      if (tm.coeff == 0.0) continue;
+      if (launches.empty() || launches.back().x != tm.x || launches.back().ids.size() == kExpectGroup)
+        launches.push_back(Launch{tm.x, {}});
+      launches.back().ids.push_back(order[k]);
+    }
+    // Results of up to 256 launches are read back per host synchronisation.
+    const size_t per_sync = 256;
+    std::vector<double> host(per_sync * kExpectGroup);
     double total = 0.0;
-    for (uint32_t t0 = 0; t0 < n_terms; t0 += batch) {
-      const uint32_t nb = std::min(batch, n_terms - t0);
-      for (uint32_t j = 0; j < nb; j++) {
-        const Term& tm = terms[t0 + j];
-        if (st->precision == ALEPH_HIP_F64)
-          launch_expect_pauli<double>(c->stream, st->d, st->n, tm.x, tm.zy, partials, res + 2 * j);
-        else
-          launch_expect_pauli<float>(c->stream, st->d, st->n, tm.x, tm.zy, partials, res + 2 * j);
-      }
-      ALEPH_HIP_CHECK(hipMemcpyAsync(host.data(), res, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-      ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
-      for (uint32_t j = 0; j < nb; j++) {
-        const double re = host[2 * j], im = host[2 * j + 1];
-        double v;
-        switch (terms[t0 + j].ny & 3) {  // Re(i^ny * (re + i im))
-          case 0: v = re; break;
-          case 1: v = -im; break;
-          case 2: v = -re; break;
-          default: v = im; break;
+    for (size_t l0 = 0; l0 < launches.size(); l0 += per_sync) {
+      const size_t nl = std::min(per_sync, launches.size() - l0);
+      for (size_t j = 0; j < nl; j++) {
+        const Launch& L = launches[l0 + j];
+        uint64_t zy[kExpectGroup];
+        uint8_t use_im[kExpectGroup];
+        for (size_t t = 0; t < L.ids.size(); t++) {
+          const Term& tm = terms[L.ids[t]];
+          zy[t] = tm.zy;
+          use_im[t] = (uint8_t)(tm.x != 0 && (__builtin_popcountll(tm.x & tm.zy) & 1));
         }
-        total += terms[t0 + j].coeff * v;
+        double* out_j = res + j * kExpectGroup;
+        if (st->precision == ALEPH_HIP_F64)
+          launch_expect_group<double>(c->stream, st->d, st->n, L.x, zy, use_im, (uint32_t)L.ids.size(), partials,
+                                      out_j);
+        else
+          launch_expect_group<float>(c->stream, st->d, st->n, L.x, zy, use_im, (uint32_t)L.ids.size(), partials,
+                                     out_j);
+      }
+      ALEPH_HIP_CHECK(hipMemcpyAsync(host.data(), res, nl * kExpectGroup * sizeof(double), hipMemcpyDeviceToHost,
+                                     c->stream));
+      ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+      for (size_t j = 0; j < nl; j++) {
+        const Launch& L = launches[l0 + j];
+        for (size_t t = 0; t < L.ids.size(); t++) {
+          const Term& tm = terms[L.ids[t]];
+          // (re, im) of sum_i conj(psi[i ^ x]) (-1)^{|i & zy|} psi[i]: the pair
+          // sums count each pair once, so they carry a factor 2.
+          double re = 0.0, im = 0.0;
+          const double acc = host[j * kExpectGroup + t];
+          if (tm.x == 0) re = acc;
+          else if (__builtin_popcountll(tm.x & tm.zy) & 1) im = 2.0 * acc;
+          else re = 2.0 * acc;
+          double v;
+          switch (tm.ny & 3) {  // Re(i^ny * (re + i im))
+            case 0: v = re; break;
+            case 1: v = -im; break;
+            case 2: v = -re; break;
+            default: v = im; break;
+          }
+          total += tm.coeff * v;
+        }
       }
     }
     *out = total;
+  });
+}
+
+aleph_hip_status aleph_hip_linear_xeb(aleph_hip_ctx* c, aleph_hip_state* st, double* out) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(out != nullptr, "null output");
+    double* partials = c->d_scratch;
+    double* res = c->d_scratch + kResOffset;
+    if (st->precision == ALEPH_HIP_F64) launch_sum_p2<double>(c->stream, st->d, st->n, partials, res);
+    else launch_sum_p2<float>(c->stream, st->d, st->n, partials, res);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(c->h_pinned, res, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (c->h_pinned[2] != 0.0) throw make_failure(ALEPH_HIP_ERR_INVALID_STATE, "non-finite amplitude", "invalid state");
+    *out = std::ldexp(c->h_pinned[0], (int)st->n) - 1.0;
   });
 }
 
@@ -1093,9 +1153,9 @@ aleph_hip_status aleph_hip_batch_set_qaoa_angles(aleph_hip_ctx* c, aleph_hip_bat
         q.betas[l] = betas[i * p + l];
       }
     }
-    const size_t n_items = b.items.size(), n_rounds = b.round_begin.size(), n_pass = b.passes.size();
+    // Angles can change the plan's structure (a zero angle drops its gate),
+    // so the captured graph is re-validated against the new plan on the next run.
     b.plan();
-    if (b.items.size() != n_items || b.round_begin.size() != n_rounds || b.passes.size() != n_pass) b.drop_graph();
     b.upload(c->stream, c->device);
   });
 }

@@ -4,7 +4,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
+#include <set>
 #include <string>
+#include <utility>
 
 #include "../../include/aleph_hip.h"
 #include "device_types.hpp"
@@ -30,5 +33,22 @@ struct Failure {
       throw f_;                                                                       \
     }                                                                                 \
   } while (0)
+
+// Raises kernel `fn`'s dynamic LDS limit to the CU's 160 KiB, once per
+// (device, kernel).  Thread-safe: plans are built and launched from several
+// host threads, and a process may drive more than one device.
+inline void allow_full_lds(const void* fn) {
+  static std::mutex mu;
+  static std::set<std::pair<int, const void*>> done;
+  int dev = 0;
+  ALEPH_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if (!done.insert({dev, fn}).second) return;
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e != hipSuccess) {
+    done.erase({dev, fn});
+    ALEPH_HIP_CHECK(e);
+  }
+}
 
 }  // namespace aleph

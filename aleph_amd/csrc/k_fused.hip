@@ -33,12 +33,7 @@ static void launch_bt(hipStream_t stream, void* state, uint32_t n, uint32_t m, u
   const uint64_t blocks = 1ull << (n - m);
   const uint32_t threads = 1u << (m - G);
   const size_t lds = n_stages > 1 ? (sizeof(C2<T>) << m) : 0;
-  static bool configured = false;
-  if (!configured) {
-    ALEPH_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused_pass<T, G, BT, FEAT, NT>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    configured = true;
-  }
+  allow_full_lds(reinterpret_cast<const void*>(&k_fused_pass<T, G, BT, FEAT, NT>));
   hipLaunchKernelGGL((k_fused_pass<T, G, BT, FEAT, NT>), dim3((uint32_t)blocks), dim3(threads), lds, stream,
                      reinterpret_cast<C2<T>*>(state), pd, stages, ops, terms);
   ALEPH_HIP_CHECK(hipGetLastError());

@@ -255,12 +255,7 @@ void launch_kq(hipStream_t st, void* state, uint32_t n, uint32_t k, const uint32
   uint64_t blocks = (groups + per_tile - 1) / per_tile;
   if (blocks > 65536) blocks = 65536;
   const size_t lds = sizeof(C2<T>) * ((size_t)dim * dim + kBlock);
-  static bool configured = false;
-  if (!configured) {
-    ALEPH_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_kq<T>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    configured = true;
-  }
+  allow_full_lds(reinterpret_cast<const void*>(&k_kq<T>));
   hipLaunchKernelGGL(k_kq<T>, dim3((uint32_t)blocks), dim3(kBlock), lds, st, (C2<T>*)state, groups, k,
                      to_perm(qubits, k), cmask, (const C2<T>*)dmat);
   ALEPH_HIP_CHECK(hipGetLastError());

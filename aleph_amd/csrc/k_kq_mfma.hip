@@ -130,12 +130,7 @@ template <typename T, int K>
 void launch_k(hipStream_t st, void* state, uint32_t n, const uint32_t* qubits, uint64_t cmask, const void* dmat) {
   constexpr int D = 1 << K;
   const size_t lds = sizeof(T) * (size_t)(2 * D / 16) * (2 * D / 4) * 64 + sizeof(uint64_t) * D;
-  static bool configured = false;
-  if (!configured) {
-    ALEPH_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_kq_mfma<T, K>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    configured = true;
-  }
+  allow_full_lds(reinterpret_cast<const void*>(&k_kq_mfma<T, K>));
   KqArgs a{};
   for (int j = 0; j < K; j++) a.msb_first[j] = a.sorted[j] = qubits[j];
   for (int i = 0; i < K; i++)

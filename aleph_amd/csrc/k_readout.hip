@@ -61,19 +61,42 @@ __device__ void block_reduce(KSum (&v)[N]) {
   }
 }
 
-template <typename T>
+// Grid-stride sweeps take kUnroll amplitudes per thread per iteration, all
+// loads issued before any arithmetic, so each wave keeps several 16-byte
+// loads in flight.
+constexpr int kUnroll = 4;
+
+// MODE 0: [sum over bit q clear, sum over bit q set, non-finite count] of p = |a|^2.
+// MODE 1: [sum of p^2, sum of p, non-finite count] (collision probability).
+template <typename T, int MODE>
 __global__ void __launch_bounds__(kThreads) k_reduce_branch(const C2<T>* __restrict__ s, uint64_t len,
                                                             uint64_t qmask, double* __restrict__ partials) {
   KSum v[3] = {{0, 0}, {0, 0}, {0, 0}};
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
-    const C2<T> a = s[i];
-    const double p = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
-    if (!isfinite(p)) {
-      v[2].s += 1.0;
-      continue;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < len; i0 += kUnroll * stride) {
+    C2<T> a[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const uint64_t i = i0 + u * stride;
+      if (i < len) a[u] = s[i];
+      else a[u].x = a[u].y = 0;
     }
-    if (i & qmask) kadd(v[1], p);
-    else kadd(v[0], p);
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const uint64_t i = i0 + u * stride;
+      const double p = (double)a[u].x * (double)a[u].x + (double)a[u].y * (double)a[u].y;
+      if (!isfinite(p)) {
+        v[2].s += 1.0;
+        continue;
+      }
+      if (MODE == 0) {
+        if (i & qmask) kadd(v[1], p);
+        else kadd(v[0], p);
+      } else {
+        kadd(v[0], p * p);
+        kadd(v[1], p);
+      }
+    }
   }
   block_reduce<3>(v);
   if (threadIdx.x == 0) {
@@ -126,6 +149,91 @@ __global__ void __launch_bounds__(kThreads) k_expect_pauli(const C2<T>* __restri
     partials[3 * blockIdx.x + 0] = v[0].s + v[0].c;
     partials[3 * blockIdx.x + 1] = v[1].s + v[1].c;
     partials[3 * blockIdx.x + 2] = 0.0;
+  }
+}
+
+// All Pauli terms that share one X mask `x` in one read of the state.
+// With z = conj(psi[i ^ x]) psi[i] and s_t(i) = (-1)^{|i & zy_t|}, term t
+// needs R_t = sum_i s_t(i) Re z(i) and I_t = sum_i s_t(i) Im z(i).  For
+// x != 0 the sum runs over pairs {i, j = i ^ x} (i with x's top bit clear),
+// each amplitude read once: z(j) = conj z(i) and s_t(j) = c_t s_t(i) with
+// c_t = (-1)^{|x & zy_t|}, so the pair adds 2 s_t(i) Re z(i) to R_t when
+// c_t = 1 (I_t gets 0) and 2 s_t(i) Im z(i) to I_t when c_t = -1 (R_t
+// gets 0).  Hence one accumulator per term: acc_t = sum s_t(i) w_t(i) with
+// w_t = Re z or Im z; the factor 2 and the placement are applied on the host.
+// For x = 0, w = |psi_i|^2 over every i.
+constexpr int kGroupTerms = (int)kExpectGroup;
+struct GroupTerms {
+  uint64_t zy[kGroupTerms];
+  uint32_t use_im;   // bit t: term t accumulates Im z
+  uint32_t count;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_expect_group(const C2<T>* __restrict__ s, uint64_t pairs, uint64_t x,
+                                                           uint32_t hbit, GroupTerms g,
+                                                           double* __restrict__ partials) {
+  double acc[kGroupTerms];
+#pragma unroll
+  for (int t = 0; t < kGroupTerms; t++) acc[t] = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t lo_mask = (1ull << hbit) - 1ull;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < pairs; k += stride) {
+    // k enumerates indices with bit hbit clear (every index when x == 0).
+    const uint64_t i = x ? (((k & ~lo_mask) << 1) | (k & lo_mask)) : k;
+    const C2<T> a = s[i];
+    double re, im;
+    if (x) {
+      const C2<T> b = s[i ^ x];
+      re = (double)b.x * (double)a.x + (double)b.y * (double)a.y;
+      im = (double)b.x * (double)a.y - (double)b.y * (double)a.x;
+    } else {
+      re = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+      im = 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < kGroupTerms; t++) {
+      if (t < (int)g.count) {
+        const double w = ((g.use_im >> t) & 1u) ? im : re;
+        acc[t] += (__popcll(i & g.zy[t]) & 1) ? -w : w;
+      }
+    }
+  }
+  // Block reduction per term: wave shuffles, then the block's waves in LDS.
+  __shared__ double sh[kGroupTerms][kThreads / 64];
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < kGroupTerms; t++) {
+    double v = acc[t];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) sh[t][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < (uint32_t)g.count) {
+    double v = 0.0;
+    for (uint32_t w = 0; w < blockDim.x / 64; w++) v += sh[threadIdx.x][w];
+    partials[(uint64_t)blockIdx.x * kGroupTerms + threadIdx.x] = v;
+  }
+}
+
+// Sums `count` rows of `width` partials column-wise into out[0..width)
+// (compensated), one workgroup.
+__global__ void __launch_bounds__(kThreads) k_reduce_columns(const double* __restrict__ partials, uint32_t count,
+                                                             uint32_t width, double* __restrict__ out) {
+  __shared__ double sh[kThreads];
+  for (uint32_t w = 0; w < width; w++) {
+    KSum v = {0, 0};
+    for (uint32_t b = threadIdx.x; b < count; b += blockDim.x) kadd(v, partials[(uint64_t)width * b + w]);
+    v = wave_reduce(v);
+    if ((threadIdx.x & 63u) == 0) sh[threadIdx.x >> 6] = v.s + v.c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      KSum acc = {0, 0};
+      for (uint32_t i = 0; i < blockDim.x / 64; i++) kadd(acc, sh[i]);
+      out[w] = acc.s + acc.c;
+    }
+    __syncthreads();
   }
 }
 
@@ -306,11 +414,43 @@ void launch_reduce_branch(hipStream_t st, const void* state, uint32_t n, int32_t
                           double* out) {
   const uint64_t len = 1ull << n;
   const uint64_t qmask = qubit >= 0 ? (1ull << qubit) : 0ull;
-  uint32_t blocks = grid_for(len);
+  uint32_t blocks = grid_for((len + kUnroll - 1) / kUnroll);
   if (blocks > kReduceBlocks) blocks = kReduceBlocks;
-  hipLaunchKernelGGL(k_reduce_branch<T>, dim3(blocks), dim3(kThreads), 0, st, (const C2<T>*)state, len, qmask,
+  hipLaunchKernelGGL((k_reduce_branch<T, 0>), dim3(blocks), dim3(kThreads), 0, st, (const C2<T>*)state, len, qmask,
                      partials);
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, st, partials, blocks, 3u, out);
+  ALEPH_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_sum_p2(hipStream_t st, const void* state, uint32_t n, double* partials, double* out) {
+  const uint64_t len = 1ull << n;
+  uint32_t blocks = grid_for((len + kUnroll - 1) / kUnroll);
+  if (blocks > kReduceBlocks) blocks = kReduceBlocks;
+  hipLaunchKernelGGL((k_reduce_branch<T, 1>), dim3(blocks), dim3(kThreads), 0, st, (const C2<T>*)state, len, 0ull,
+                     partials);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, st, partials, blocks, 3u, out);
+  ALEPH_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_expect_group(hipStream_t st, const void* state, uint32_t n, uint64_t xmask, const uint64_t* zy,
+                         const uint8_t* use_im, uint32_t count, double* partials, double* out) {
+  if (count == 0 || count > (uint32_t)kGroupTerms) throw Failure{ALEPH_HIP_ERR_INVALID_ARGUMENT, "group size"};
+  GroupTerms g{};
+  for (uint32_t t = 0; t < count; t++) {
+    g.zy[t] = zy[t];
+    if (use_im[t]) g.use_im |= 1u << t;
+  }
+  g.count = count;
+  uint32_t hbit = 0;
+  if (xmask) hbit = 63 - __builtin_clzll(xmask);
+  const uint64_t pairs = xmask ? (1ull << (n - 1)) : (1ull << n);
+  uint32_t blocks = grid_for(pairs);
+  if (blocks > kReduceBlocks) blocks = kReduceBlocks;
+  hipLaunchKernelGGL(k_expect_group<T>, dim3(blocks), dim3(kThreads), 0, st, (const C2<T>*)state, pairs, xmask, hbit,
+                     g, partials);
+  hipLaunchKernelGGL(k_reduce_columns, dim3(1), dim3(kThreads), 0, st, partials, blocks, (uint32_t)kGroupTerms, out);
   ALEPH_HIP_CHECK(hipGetLastError());
 }
 
@@ -369,6 +509,9 @@ void launch_sample(hipStream_t st, const void* state, uint32_t n, double* block_
 
 #define ALEPH_INST(T)                                                                                           \
   template void launch_reduce_branch<T>(hipStream_t, const void*, uint32_t, int32_t, double*, double*);         \
+  template void launch_sum_p2<T>(hipStream_t, const void*, uint32_t, double*, double*);                         \
+  template void launch_expect_group<T>(hipStream_t, const void*, uint32_t, uint64_t, const uint64_t*,            \
+                                       const uint8_t*, uint32_t, double*, double*);                             \
   template void launch_collapse<T>(hipStream_t, void*, uint32_t, uint32_t, int32_t, double);                    \
   template void launch_expect_pauli<T>(hipStream_t, const void*, uint32_t, uint64_t, uint64_t, double*, double*); \
   template void launch_marginal<T>(hipStream_t, const void*, uint32_t, uint32_t, const uint32_t*, double*);      \

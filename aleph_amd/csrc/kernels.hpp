@@ -95,10 +95,22 @@ void launch_mlp_adam(hipStream_t s, float* params, const float* grads, float* m,
 // ---- readout (k_readout.hip) ----
 // Result layout in `out` (device, 3 doubles): [sum of |a|^2 over bit q clear,
 // sum over bit q set, non-finite count]; qubit < 0 puts everything in [0].  `partials` needs 3 * kReduceBlocks doubles.
-constexpr uint32_t kReduceBlocks = 1024;
+constexpr uint32_t kReduceBlocks = 4096;
 template <typename T>
 void launch_reduce_branch(hipStream_t s, const void* state, uint32_t n, int32_t qubit, double* partials,
                           double* out);
+// out[0] = sum of |a|^4, out[1] = sum of |a|^2, out[2] = non-finite count (same partials as above).
+template <typename T>
+void launch_sum_p2(hipStream_t s, const void* state, uint32_t n, double* partials, double* out);
+// Up to kExpectGroup Pauli terms sharing X mask `xmask`, one read of the state:
+// out[t] = sum_i (-1)^{|i & zy[t]|} w_t(i), with w_t = |psi_i|^2 when xmask == 0 and
+// otherwise Re (use_im[t] == 0) or Im of conj(psi[i ^ xmask]) psi[i] over the
+// indices i with xmask's top bit clear.  `partials` needs kExpectGroup * kReduceBlocks
+// doubles; `out` kExpectGroup doubles.
+constexpr uint32_t kExpectGroup = 16;
+template <typename T>
+void launch_expect_group(hipStream_t s, const void* state, uint32_t n, uint64_t xmask, const uint64_t* zy,
+                         const uint8_t* use_im, uint32_t count, double* partials, double* out);
 template <typename T>
 void launch_collapse(hipStream_t s, void* state, uint32_t n, uint32_t qubit, int32_t outcome, double scale);
 // <psi| P |psi> pieces: out[0] = Re, out[1] = Im of sum conj(psi[i^x]) * (-1)^{|i & zy|} psi[i].

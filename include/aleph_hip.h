@@ -261,10 +261,15 @@ aleph_hip_status aleph_hip_expectation(aleph_hip_ctx* ctx, aleph_hip_state* st, 
 /* sum_t coeffs[t] * <psi|P_t|psi> for n_terms Pauli strings; term t acts on
  * qubits[offsets[t] .. offsets[t+1]) with paulis[...] (codes as above).
  * Replaces Backend::expectation_pauli_sum (crates/aleph-backend/src/lib.rs:422-432):
- * one validation, one read of the state per term, one host synchronisation. */
+ * one validation; terms sharing an X mask are evaluated together, up to 16 per read of the
+ * state (a diagonal Hamiltonian of T terms costs ceil(T / 16) reads). */
 aleph_hip_status aleph_hip_expectation_sum(aleph_hip_ctx* ctx, aleph_hip_state* st, uint32_t n_terms,
                                            const double* coeffs, const uint32_t* offsets, const uint32_t* qubits,
                                            const uint8_t* paulis, double* out);
+/* Linear cross-entropy benchmark of the exact state, 2^n * sum_x |psi_x|^4 - 1, accumulated in
+ * FP64 in one read of the state: benches/src/lib.rs:29 linear_xeb (computed there on the host
+ * from the amplitudes). */
+aleph_hip_status aleph_hip_linear_xeb(aleph_hip_ctx* ctx, aleph_hip_state* st, double* out);
 /* Marginal over `qubits` (qubits[0] = LSB of the output index), 2^n_qubits doubles. */
 aleph_hip_status aleph_hip_probabilities(aleph_hip_ctx* ctx, aleph_hip_state* st, const uint32_t* qubits,
                                          uint32_t n_qubits, double* out);

@@ -95,3 +95,25 @@ def random_unitary(dim: int, rng: np.random.Generator) -> np.ndarray:
 
 def max_abs(a: np.ndarray, b: np.ndarray) -> float:
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b)))) if np.size(a) else 0.0
+
+
+def qaoa_p1_edge_expectation(gamma: float, beta: float, d_u: int, d_v: int, triangles: int) -> float:
+    """<(1 - Z_u Z_v)/2> after one QAOA layer on MaxCut, closed form of Wang,
+    Hadfield, Jiang & Rieffel, PRA 97, 022304 (2018), Theorem 1, for the
+    ansatz of aleph_amd.workloads.qaoa_maxcut (RZZ(gamma) = exp(-i gamma Z Z / 2),
+    RX(beta) = exp(-i beta X / 2)).  In the paper's convention (e^{-i g C},
+    e^{-i b B}) this circuit is g = gamma, b = -beta / 2 up to a global phase;
+    d_u, d_v count the other neighbours of u and v, `triangles` the common ones."""
+    g, b = gamma, -0.5 * beta
+    return (0.5 + 0.25 * math.sin(4 * b) * math.sin(g) * (math.cos(g) ** d_u + math.cos(g) ** d_v)
+            - 0.25 * math.sin(2 * b) ** 2 * math.cos(g) ** (d_u + d_v - 2 * triangles)
+            * (1 - math.cos(2 * g) ** triangles))
+
+
+def qaoa_p1_cut_expectation(n: int, edges, gamma: float, beta: float) -> float:
+    nb = [set() for _ in range(n)]
+    for u, v in edges:
+        nb[u].add(v)
+        nb[v].add(u)
+    return sum(qaoa_p1_edge_expectation(gamma, beta, len(nb[u]) - 1, len(nb[v]) - 1, len(nb[u] & nb[v]))
+               for u, v in edges)

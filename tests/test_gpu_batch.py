@@ -15,7 +15,7 @@ from aleph_amd import _lib as L
 from aleph_amd import workloads as W
 from aleph_amd.backend import DeviceFailure
 from aleph_amd.batch import HipBatch, maxcut_value
-from tests.helpers import PARITY_TOL, max_abs
+from tests.helpers import GOLDEN, PARITY_TOL, max_abs, qaoa_p1_cut_expectation
 from tests.test_oracle import random_circuit
 
 pytestmark = pytest.mark.gpu
@@ -144,3 +144,72 @@ def test_batch_rejects_bad_input(hip):
         HipBatch.qaoa(hip, [(4, [(0, 1)])], [[0.1]], [[0.2]])          # n < 5
     with pytest.raises(Exception):
         HipBatch.qaoa(hip, [(6, [(0, 9)])], [[0.1]], [[0.2]])          # edge out of range
+
+
+# ---------------------------------------------------------------- the QAOA workload at its own sizes
+
+
+@pytest.mark.parametrize("mode", [L.BATCH_PERSISTENT, L.BATCH_ROUNDS])
+def test_qaoa_p4_up_to_n20_matches_oracle(hip, orc, mode):
+    """p = 4 with n even in [12, 20] (the workload's depth, and sizes the
+    oracle finishes in seconds), FP32 amplitudes and FP64 <C>."""
+    graphs, gammas, betas = W.qaoa_batch(10, seed=404 + mode, n_min=12, n_max=20, p=4)
+    assert max(n for n, _ in graphs) == 20
+    b = HipBatch.qaoa(hip, graphs, gammas, betas, precision=32, mode=mode)
+    b.run()
+    check_qaoa_batch(orc, b, graphs, gammas, betas, FP32_TOL)
+    b.free()
+
+
+def test_qaoa_4096_instances_p1_closed_form(hip):
+    """The whole workload shape (4096 seeded 3-regular instances, n even in
+    [12, 24], FP32, one batch) with p = 1 angles: every instance's <C> equals
+    the analytic p = 1 expectation (tests/helpers.qaoa_p1_cut_expectation,
+    pinned against brute force in tests/test_qaoa_closed_form.py)."""
+    graphs, gammas, betas = W.qaoa_batch(4096, seed=2024, p=1)
+    b = HipBatch.qaoa(hip, graphs, gammas, betas, precision=32, mode=L.BATCH_ROUNDS)
+    b.run()
+    got = b.expect_maxcut(graphs)
+    b.free()
+    want = np.array([qaoa_p1_cut_expectation(n, e, gammas[i, 0], betas[i, 0]) for i, (n, e) in enumerate(graphs)])
+    err = np.abs(got - want) / np.array([len(e) for _, e in graphs])
+    print(f"4096 instances p=1: max |<C> - closed form| per edge {err.max():.2e}, mean <C> {got.mean():.4f}",
+          flush=True)
+    assert err.max() < 1e-5
+
+
+@pytest.mark.parametrize("name", ["qaoa_n6", "qaoa_n10", "qaoa_n14"])
+def test_qaoa_reference_graphs(hip, orc, name):
+    """The reference's committed graphs (scripts/qaoa/graphs): p = 4 batch vs the
+    oracle, and <C> never above the recorded maximum cut."""
+    edges = [tuple(int(t) for t in line.split()) for line in (GOLDEN / "qaoa" / f"{name}.edges").read_text().split("\n")
+             if line.strip() and not line.startswith("#")]
+    n = 1 + max(max(e) for e in edges)
+    maxcut = float((GOLDEN / "qaoa" / f"{name}.maxcut").read_text().split()[0])
+    rng = np.random.default_rng(len(edges))
+    gammas, betas = rng.uniform(0, np.pi, (1, 4)), rng.uniform(0, np.pi, (1, 4))
+    if n < 5:
+        pytest.skip("batches take n >= 5")
+    b = HipBatch.qaoa(hip, [(n, edges)], gammas, betas, precision=64)
+    b.run()
+    check_qaoa_batch(orc, b, [(n, edges)], gammas, betas, PARITY_TOL)
+    assert b.expect_maxcut([(n, edges)])[0] <= maxcut + 1e-9
+    b.free()
+
+
+def test_rounds_replay_after_a_zero_angle_changes_the_plan(hip, orc):
+    """A zero gamma drops an instance's cost layer, so its pass count and the
+    per-round item ranges change while the totals may not: the captured graph
+    must be re-captured, and every replay must still equal the oracle."""
+    graphs, gammas, betas = small_qaoa(3, seed=77, n_min=10, n_max=12, p=2)
+    b = HipBatch.qaoa(hip, graphs, gammas, betas, precision=64, mode=L.BATCH_ROUNDS, local_qubits=8)
+    b.run()
+    check_qaoa_batch(orc, b, graphs, gammas, betas, PARITY_TOL)
+    gammas = gammas.copy()
+    gammas[0, :] = 0.0
+    gammas[1, 0] = 0.0
+    b.set_qaoa_angles(gammas, betas)
+    for _ in range(2):
+        b.run()
+        check_qaoa_batch(orc, b, graphs, gammas, betas, PARITY_TOL)
+    b.free()

@@ -29,31 +29,32 @@ def _run(b, c, psi, **kw):
     return out
 
 
+# (name, seed of the random-circuit generator, builder, planner options)
 CASES = [
-    ("brickwall", lambda rng: W.brickwall(18, 10, seed=4), {}),
-    ("hea", lambda rng: W.hea(16, 4, seed=2), {}),
-    ("qaoa", lambda rng: W.qaoa_maxcut(14, W.random_3_regular(14, 5), [0.4, 1.1], [0.7, 0.2]), {}),
-    ("random_ctl", lambda rng: random_circuit(14, 120, rng, kq=False), {}),
-    ("random_m10", lambda rng: random_circuit(13, 120, rng, kq=False), {"local_qubits": 10}),
-    ("random_g5", lambda rng: random_circuit(14, 120, rng, kq=False), {"reg_bits": 5, "local_qubits": 12}),
-    ("brickwall_nt", lambda rng: W.brickwall(17, 8, seed=9), {"nontemporal": True, "fuse_2q": False}),
-    ("hea_m10", lambda rng: W.hea(15, 4, seed=6), {"local_qubits": 10}),
+    ("brickwall", 11, lambda rng: W.brickwall(18, 10, seed=4), {}),
+    ("hea", 12, lambda rng: W.hea(16, 4, seed=2), {}),
+    ("qaoa", 13, lambda rng: W.qaoa_maxcut(14, W.random_3_regular(14, 5), [0.4, 1.1], [0.7, 0.2]), {}),
+    ("random_ctl", 14, lambda rng: random_circuit(14, 120, rng, kq=False), {}),
+    ("random_m10", 15, lambda rng: random_circuit(13, 120, rng, kq=False), {"local_qubits": 10}),
+    ("random_g5", 16, lambda rng: random_circuit(14, 120, rng, kq=False), {"reg_bits": 5, "local_qubits": 12}),
+    ("brickwall_nt", 17, lambda rng: W.brickwall(17, 8, seed=9), {"nontemporal": True, "fuse_2q": False}),
+    ("hea_m10", 18, lambda rng: W.hea(15, 4, seed=6), {"local_qubits": 10}),
 ]
 
 
 @pytest.mark.parametrize("precision,tol", [(64, PARITY_TOL), (32, 1e-5)])
-@pytest.mark.parametrize("name,make,kw", CASES, ids=[c[0] for c in CASES])
-def test_generated_kernels_equal_compiled_kernels(precision, tol, name, make, kw):
-    rng = np.random.default_rng(abs(hash(name)) % 1000)
+@pytest.mark.parametrize("name,seed,make,kw", CASES, ids=[c[0] for c in CASES])
+def test_generated_kernels_equal_compiled_kernels(precision, tol, name, seed, make, kw):
+    rng = np.random.default_rng(seed)
     c = make(rng)
     psi = W.gaussian_state(c.num_qubits, 7)
     b = HipSvBackend(seed=3, precision=precision)
     jit = _run(b, c, psi, jit=True, **kw)
     ref = _run(b, c, psi, **kw)
     # Same arithmetic up to the compiler's FMA contraction choices.
-    assert max_abs(jit, ref) <= (1e-14 if precision == 64 else 1e-6)
+    assert max_abs(jit, ref) <= (1e-14 if precision == 64 else 1e-6), f"{name} seed={seed}"
     o = OracleSv(cap=30)
     st = o.allocate(c.num_qubits)
     st.load(psi)
     o.run(c, st)
-    assert max_abs(jit, st.amplitudes()) < tol
+    assert max_abs(jit, st.amplitudes()) < tol, f"{name} seed={seed}"
