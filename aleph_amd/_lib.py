@@ -147,6 +147,7 @@ SIGNATURES = {
     "aleph_hip_measure_with_uniform": (i32, [P, P, u32, f64, PI32, PI32]),
     "aleph_hip_sample": (i32, [P, P, u32, PU64]),
     "aleph_hip_sample_with_uniforms": (i32, [P, P, PF64, u32, PU64]),
+    "aleph_hip_sample_with_uniforms_noalloc": (i32, [P, P, PF64, u32, PU64]),
     "aleph_hip_expectation": (i32, [P, P, f64, PU32, PU8, u32, PF64]),
     "aleph_hip_expectation_sum": (i32, [P, P, u32, PF64, PU32, PU32, PU8, PF64]),
     "aleph_hip_linear_xeb": (i32, [P, P, PF64]),

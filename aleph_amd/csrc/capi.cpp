@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "batch.hpp"
@@ -23,6 +24,25 @@
 
 using namespace aleph;
 
+// Device memory of the allocation-free sampler
+// (aleph_hip_sample_with_uniforms_noalloc): the block sums and their prefix
+// for a state of up to kNoAllocMaxQubits qubits, the scan's total, and up to
+// kNoAllocMaxShots uniforms and results: 17 MiB.  One copy per device, held
+// until the process ends.  In host code `scratch` only names the symbol; its
+// device address comes from hipGetSymbolAddress (noalloc_scratch_address).
+namespace aleph {
+constexpr uint32_t kNoAllocMaxQubits = 32;
+constexpr uint32_t kNoAllocMaxShots = 1u << 16;
+constexpr size_t kNoAllocBlocks = size_t(1) << (kNoAllocMaxQubits - kSampleBlockBits);
+constexpr size_t kNoAllocSumsOff = 0;                                          // 2 * kNoAllocBlocks doubles
+constexpr size_t kNoAllocMiscOff = kNoAllocSumsOff + 2 * kNoAllocBlocks * sizeof(double);  // 64 doubles
+constexpr size_t kNoAllocUniOff = kNoAllocMiscOff + 64 * sizeof(double);       // kNoAllocMaxShots doubles
+constexpr size_t kNoAllocOutOff = kNoAllocUniOff + kNoAllocMaxShots * sizeof(double);  // kNoAllocMaxShots u64
+constexpr size_t kNoAllocBytes = kNoAllocOutOff + kNoAllocMaxShots * sizeof(uint64_t);
+}  // namespace aleph
+
+static __device__ __align__(256) unsigned char scratch[aleph::kNoAllocBytes];
+
 struct aleph_hip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -33,6 +53,7 @@ struct aleph_hip_ctx {
   size_t scratch_bytes = 0;
   double* h_pinned = nullptr;       // 64 doubles of pinned readback
   float* d_model = nullptr;         // cost model of the learned planner (kMlpParams floats) or null
+  void* noalloc_scratch = nullptr;  // this device's copy of `scratch`
   std::vector<aleph_hip_state*> states;  // live states, detached from the context when it is destroyed
 };
 
@@ -160,6 +181,22 @@ void check_distinct(const uint32_t* qs, uint32_t k, const aleph_hip_state* st, c
 
 size_t amp_bytes(const aleph_hip_state* st) { return st->precision == ALEPH_HIP_F64 ? 16 : 8; }
 
+// Device address of `scratch` on the current device `dev`, looked up once per
+// device (each device holds its own copy of the array at its own address).
+void* noalloc_scratch_address(int dev) {
+  static std::mutex mu;
+  static std::vector<void*> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((size_t)dev >= addr.size()) addr.resize((size_t)dev + 1, nullptr);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    ALEPH_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Failure{ALEPH_HIP_ERR_DEVICE_FAILURE, "no device address for the sampler scratch"};
+    addr[dev] = p;
+  }
+  return addr[dev];
+}
+
 // Σ|ψ|² over bit q clear / set, non-finite count (q < 0: everything in [0]).
 void reduce(aleph_hip_ctx* c, aleph_hip_state* st, int32_t q, double out[3]) {
   double* partials = c->d_scratch;
@@ -285,6 +322,7 @@ aleph_hip_status aleph_hip_ctx_create(int32_t ordinal, uint64_t seed, aleph_hip_
     ALEPH_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&c->d_scratch), kScratchBytes));
     c->scratch_bytes = kScratchBytes;
     ALEPH_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_pinned), 64 * sizeof(double), hipHostMallocDefault));
+    c->noalloc_scratch = noalloc_scratch_address(c->device);
   });
   if (st != ALEPH_HIP_OK) {
     aleph_hip_ctx_destroy(c);
@@ -844,6 +882,38 @@ aleph_hip_status aleph_hip_sample_with_uniforms(aleph_hip_ctx* c, aleph_hip_stat
                                                 uint64_t* out) {
   if (shots && !u) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
   return sample_impl(c, st, u, shots, out);
+}
+
+aleph_hip_status aleph_hip_sample_with_uniforms_noalloc(aleph_hip_ctx* c, aleph_hip_state* st, const double* u,
+                                                        uint32_t shots, uint64_t* out) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    need(shots == 0 || (u != nullptr && out != nullptr), "null argument");
+    if (st->n > kNoAllocMaxQubits || shots > kNoAllocMaxShots) {
+      Failure f{ALEPH_HIP_ERR_INVALID_ARGUMENT,
+                "allocation-free sampling takes at most 32 qubits and 65536 shots per call"};
+      f.kind = "argument";
+      throw f;
+    }
+    need(c->noalloc_scratch != nullptr, "sampler scratch not set up");
+    validate(c, st, nullptr, nullptr, -1);   // context scratch, allocated with the context
+    if (shots == 0) return;
+    // Every region is written before it is read (block sums, their prefix and
+    // total by the first two kernels, the uniforms by the copy, every result
+    // slot by the shot kernel), so nothing needs clearing between calls.
+    unsigned char* base = static_cast<unsigned char*>(c->noalloc_scratch);
+    double* sums = reinterpret_cast<double*>(base + kNoAllocSumsOff);
+    double* misc = reinterpret_cast<double*>(base + kNoAllocMiscOff);
+    double* du = reinterpret_cast<double*>(base + kNoAllocUniOff);
+    uint64_t* dout = reinterpret_cast<uint64_t*>(base + kNoAllocOutOff);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(du, u, shots * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (st->precision == ALEPH_HIP_F64)
+      launch_sample<double>(c->stream, st->d, st->n, sums, misc, du, shots, dout);
+    else
+      launch_sample<float>(c->stream, st->d, st->n, sums, misc, du, shots, dout);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(out, dout, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
 }
 
 aleph_hip_status aleph_hip_expectation(aleph_hip_ctx* c, aleph_hip_state* st, double coeff, const uint32_t* qubits,

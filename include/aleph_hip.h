@@ -254,6 +254,16 @@ aleph_hip_status aleph_hip_measure_with_uniform(aleph_hip_ctx* ctx, aleph_hip_st
 aleph_hip_status aleph_hip_sample(aleph_hip_ctx* ctx, aleph_hip_state* st, uint32_t shots, uint64_t* out);
 aleph_hip_status aleph_hip_sample_with_uniforms(aleph_hip_ctx* ctx, aleph_hip_state* st, const double* uniforms,
                                                 uint32_t shots, uint64_t* out);
+/* No counterpart in the reference: aleph_hip_sample_with_uniforms for callers that cannot
+ * allocate device memory (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels and the same results, but the block sums, uniforms and shot
+ * indices live in a device array of the library (17 MiB per device, set up by
+ * aleph_hip_ctx_create and held until the process ends): no allocation in the call.  States
+ * of at most 32 qubits and at most 65536 shots per call, else ALEPH_HIP_ERR_INVALID_ARGUMENT.
+ * Every call on one device shares that array, so calls that overlap on one device (from
+ * several contexts or threads) race. */
+aleph_hip_status aleph_hip_sample_with_uniforms_noalloc(aleph_hip_ctx* ctx, aleph_hip_state* st,
+                                                        const double* uniforms, uint32_t shots, uint64_t* out);
 /* paulis[i]: 0 = I, 1 = X, 2 = Y, 3 = Z on qubits[i]. */
 aleph_hip_status aleph_hip_expectation(aleph_hip_ctx* ctx, aleph_hip_state* st, double coefficient,
                                        const uint32_t* qubits, const uint8_t* paulis, uint32_t n_terms,
