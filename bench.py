@@ -8,8 +8,10 @@ kernel generated for that pass and compiled with hiprtc at plan creation
 (`--no-jit`: the compiled op-loop kernel).  Also reported on the
 same line: the FP32 run of the same circuit, the n=30 QFT on a Gaussian
 state, the event-timed roofline of the fused pass kernel, and the CPU
-oracle (a C++ restatement of the reference CPU `sv` backend) timed on a
-bounded sample of the same circuit on this host's cores.
+oracle (a C++ restatement of the reference CPU `sv` path and its final-state
+pipeline) running the same circuit whole on this host's cores, with
+single-thread runs of two reference corpus circuits beside their published
+times.
 
 `python bench.py --gpus 1 --steps K --warmup W` prints ONE JSON line.
 """
@@ -55,7 +57,6 @@ def parse():
     ap.add_argument("--max-ops", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the FP32 and QFT legs")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline time budget")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 traffic measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched QAOA leg")
     ap.add_argument("--batch-count", type=int, default=4096)
@@ -98,48 +99,83 @@ def time_circuit(backend, circuit, steps: int, warmup: int, **plan_kw):
     return ms, stats, launch_ms, kinds, norm, st
 
 
-def cpu_baseline(circuit, budget_s: float):
+# Published single-thread times of the reference's CPU `sv` path
+# (run_optimized, AMD EPYC 8124P; docs/perf/phase4.md:27, 44).
+PUBLISHED_1T_MS = {"qft_n20": 320.03, "sycamore_n20_d20": 6028.89}
+
+
+def _cpu_threads() -> int:
+    """The host threads the CPU leg may use: OMP_NUM_THREADS when set (the
+    GPU hosts cap a job at their CPU share), else every visible core."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    n = os.cpu_count() or 1
+    return max(1, min(n, int(env))) if env.isdigit() and int(env) > 0 else n
+
+
+def _cpu_run(circuit, label: str) -> dict:
     """The reference's CPU final-state path restated (oracle/pipeline.py:
-    Fuse1qRuns, Fuse2q, FuseKq(4), TileBlock(15), then the oracle's
-    OpenMP kernels per instruction), timed on this host's cores on a
-    bounded sample: instructions run in program order until the budget is
-    spent, and the whole list is extrapolated from the mean time of each
-    instruction kind (kind = gate name and width, or tiled block)."""
+    RelabelQubits, CancelInversePairs, Fuse1qRuns, Fuse2q, FuseKq(4),
+    TileBlock(15), as final_state_cpu_pipeline, crates/aleph-ir/src/passes/
+    mod.rs:192-200), then the oracle's OpenMP kernels per instruction, the
+    whole list from |0...0> to the final (un-permuted) state: planning,
+    execution and allocation timed separately."""
     from oracle import oracle as orc
     from oracle import pipeline as P
-    threads = os.cpu_count() or 1
-    orc.set_threads(threads)
-    instrs = P.cpu_pipeline(circuit)
-
-    def key(i):
-        return "tiled" if isinstance(i, P.Tiled) else f"{i.gate.name}/{len(i.qubits)}"
-
-    o = orc.OracleSv()
-    t_alloc = time.perf_counter()
-    st = orc.OracleState(circuit.num_qubits, cap=64)
-    alloc_s = time.perf_counter() - t_alloc
-    times = {}
-    done = 0
+    t = time.perf_counter()
+    instrs, perm = P.cpu_pipeline(circuit)
+    plan_s = time.perf_counter() - t
+    o = orc.OracleSv(cap=64)
+    t = time.perf_counter()
+    st = o.allocate(circuit.num_qubits)
+    alloc_s = time.perf_counter() - t
     t0 = time.perf_counter()
-    while done < len(instrs) and time.perf_counter() - t0 < budget_s:
-        t = time.perf_counter()
-        P.apply(o, st, [instrs[done]])
-        times.setdefault(key(instrs[done]), []).append(time.perf_counter() - t)
-        done += 1
-    el = time.perf_counter() - t0
-    overall = el / max(done, 1)
-    est = sum(float(np.mean(times[key(i)])) if key(i) in times else overall for i in instrs)
+    last = t0
+    for i, ins in enumerate(instrs):
+        P.apply(o, st, [ins], perm=None)
+        now = time.perf_counter()
+        if now - last > 20.0:
+            log(f"bench: cpu {label}: {i + 1}/{len(instrs)} instructions, {now - t0:.0f} s")
+            last = now
+    if perm is not None:
+        o.unpermute_state(st, perm)
+    exec_s = time.perf_counter() - t0
     del st
-    kinds = {k: len(v) for k, v in times.items()}
+    return {"exec_s": exec_s, "plan_s": plan_s, "alloc_s": alloc_s, "gates": len(circuit.gates()),
+            "instructions": len(instrs), "relabelled": perm is not None}
+
+
+def cpu_baseline(circuit) -> dict:
+    """The benchmark circuit run whole on the host's cores (measured, not
+    extrapolated), plus single-thread runs of the reference's own corpus
+    circuits qft_n20 and sycamore_n20_d20 beside their published times."""
+    from oracle import oracle as orc
+    from aleph_amd import workloads as W
+    threads = _cpu_threads()
+    orc.set_threads(threads)
+    full = _cpu_run(circuit, f"n={circuit.num_qubits}")
+    log(f"bench: cpu full run {full['exec_s']:.1f} s on {threads} threads")
+    calib = {}
+    orc.set_threads(1)
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "corpus")
+    for name, pub in PUBLISHED_1T_MS.items():
+        r = _cpu_run(W.load_qasm(os.path.join(golden, f"{name}.qasm")), name)
+        calib[name] = {"ms_1thread": r["exec_s"] * 1e3, "published_ms_1thread": pub,
+                       "ratio_to_published": r["exec_s"] * 1e3 / pub, "instructions": r["instructions"]}
+        log(f"bench: cpu {name} 1 thread {r['exec_s'] * 1e3:.0f} ms (published {pub} ms)")
+    orc.set_threads(threads)
     return {
-        "value": est,
+        "value": full["exec_s"],
         "unit": "s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"reference CPU final-state pipeline restated (Fuse1qRuns, Fuse2q, FuseKq(4), TileBlock(15)): "
-                   f"{len(circuit.gates())} gates -> {len(instrs)} instructions; the first {done} run on the same "
-                   f"n={circuit.num_qubits} state in {el:.1f} s ({kinds}), whole list extrapolated per "
-                   f"instruction kind; allocation {alloc_s:.1f} s not included"),
+        "sample": (f"the whole circuit, measured: reference CPU final-state pipeline restated (RelabelQubits, "
+                   f"CancelInversePairs, Fuse1qRuns, Fuse2q, FuseKq(4), TileBlock(15)) turns {full['gates']} gates "
+                   f"into {full['instructions']} instructions (relabelled: {full['relabelled']}), all applied to one "
+                   f"n={circuit.num_qubits} state from |0...0> on {threads} threads; planning "
+                   f"{full['plan_s']:.2f} s and allocation {full['alloc_s']:.1f} s not included"),
+        "plan_s": full["plan_s"],
+        "alloc_s": full["alloc_s"],
+        "calibration_1thread": calib,
     }
 
 
@@ -355,7 +391,7 @@ def main():
             out["roofline"]["traffic_detail"] = tr
     if not args.no_cpu:
         log("bench: cpu baseline")
-        cpu = cpu_baseline(circuit, args.cpu_seconds)
+        cpu = cpu_baseline(circuit)
         out["cpu_baseline"] = cpu
         out["speedup_vs_cpu"] = cpu["value"] / (ms / 1e3)
     print(json.dumps(out), flush=True)

@@ -22,11 +22,19 @@ per-gate path:
 * TileBlock (passes/tile_block.rs:15-83): maximal runs (>= 2) of gates of
   arity <= 2 whose targets are all < 15 are applied tile-major.
 
-RelabelQubits and CancelInversePairs are left out: the benchmark circuits
-contain no adjacent inverse pairs, and relabelling only permutes qubit
-indices (cache locality), which does not change the work counted here.
+* RelabelQubits (passes/relabel.rs:19-95): qubits ranked by traffic (gate
+  appearances) take the low bit positions, committed only when that
+  strictly increases the number of tile-confinable gates; the final state
+  is un-permuted (perm[logical] = physical) afterwards;
+* CancelInversePairs (passes/cancel.rs): a gate is dropped together with
+  the live gate that is the most recent one on every qubit of its support,
+  when both act on the same targets (in order) and controls (as a set) and
+  one is the other's inverse (Gate::inverse, kinds.rs:506-555).
+
 Correctness of the restatement is pinned by the reference's own criterion,
-pipeline output == gate-by-gate output (tests/test_oracle.py).
+pipeline output == gate-by-gate output (tests/test_oracle.py,
+tests/test_cpu_pipeline.py), and the two new passes by the reference's own
+unit cases (cancel.rs tests, relabel.rs net-win guard).
 """
 
 from __future__ import annotations
@@ -52,6 +60,129 @@ def _mat(g: GateInstance) -> np.ndarray:
 
 def _plain(g) -> bool:
     return isinstance(g, GateInstance) and not g.controls
+
+
+def _used_qubits(ins):
+    if isinstance(ins, GateInstance):
+        return list(ins.qubits) + list(ins.controls)
+    if isinstance(ins, DiagonalPhase):
+        m = 0
+        for t in ins.terms:
+            for c in t.conds:
+                m |= int(c)
+        return [q for q in range(64) if (m >> q) & 1]
+    return list(getattr(ins, "qubits", ()))
+
+
+def _inverse_gate(g: Gate):
+    """Gate::inverse (crates/aleph-core/src/gate/kinds.rs:506-555)."""
+    n = g.name
+    if n in ("H", "X", "Y", "Z", "Cnot", "Cz", "Swap", "Toffoli", "Ccz"):
+        return g
+    pairs = {"S": "Sdg", "Sdg": "S", "T": "Tdg", "Tdg": "T", "Iswap": "IswapDg", "IswapDg": "Iswap"}
+    if n in pairs:
+        return Gate(pairs[n])
+    if n in ("Rx", "Ry", "Rz", "Phase", "CRx", "CRy", "CRz"):
+        return Gate(n, (-g.params[0],))
+    if n == "U3":
+        return Gate("U3", (-g.params[0], -g.params[2], -g.params[1]))
+    if n in ("Unitary1q", "Unitary2q"):
+        return Gate.unitary(np.conj(np.asarray(g.matrix)).T)
+    if n == "UnitaryKq":
+        return Gate.unitary_kq(np.conj(np.asarray(g.matrix)).T)
+    if n in ("Unitary1qDiag", "Unitary2qDiag"):
+        return Gate.diag(np.conj(np.asarray(g.matrix)))
+    return None
+
+
+def _same_gate(a: Gate, b: Gate) -> bool:
+    if a is None or b is None or a.name != b.name or tuple(a.params) != tuple(b.params):
+        return False
+    if a.matrix is None or b.matrix is None:
+        return a.matrix is None and b.matrix is None
+    return np.array_equal(np.asarray(a.matrix), np.asarray(b.matrix))
+
+
+def cancel_inverse_pairs(instrs):
+    """CancelInversePairs (crates/aleph-ir/src/passes/cancel.rs): one forward
+    pass with a stack of live instruction indices per qubit.  Non-gate
+    instructions are pushed and never cancel, so nothing cancels across them."""
+    result, dead, live = [], [], {}
+    for ins in instrs:
+        sup = _used_qubits(ins)
+        if not isinstance(ins, GateInstance):
+            k = len(result)
+            result.append(ins)
+            dead.append(False)
+            for q in sup:
+                live.setdefault(q, []).append(k)
+            continue
+        cand, shared = None, True
+        for q in sup:
+            st = live.get(q)
+            if not st:
+                shared = False
+                break
+            if cand is None:
+                cand = st[-1]
+            elif cand != st[-1]:
+                shared = False
+                break
+        prev = result[cand] if (shared and cand is not None) else None
+        if isinstance(prev, GateInstance) and list(prev.qubits) == list(ins.qubits) \
+                and sorted(prev.controls) == sorted(ins.controls) and _same_gate(prev.gate, _inverse_gate(ins.gate)):
+            dead[cand] = True
+            for q in sup:
+                live[q].pop()
+            continue
+        k = len(result)
+        result.append(ins)
+        dead.append(False)
+        for q in sup:
+            live.setdefault(q, []).append(k)
+    return [r for r, d in zip(result, dead) if not d]
+
+
+def _confinable(instrs, tile_bits, remap):
+    return sum(1 for ins in instrs if isinstance(ins, GateInstance) and len(ins.qubits) <= 2
+               and ins.gate.name != "UnitaryKq" and all(remap(q) < tile_bits for q in ins.qubits))
+
+
+def relabel_qubits(instrs, n: int, tile_bits: int = TILE_BITS):
+    """RelabelQubits (crates/aleph-ir/src/passes/relabel.rs:19-95).  Returns
+    (instructions, perm) with perm[logical] = physical, or perm None when the
+    relabelling does not strictly increase the tile-confinable gate count."""
+    if n <= 1:
+        return instrs, None
+    score = [0] * n
+    for ins in instrs:
+        for q in _used_qubits(ins):
+            score[q] += 1
+    order = sorted(range(n), key=lambda q: (-score[q], q))
+    phys = [0] * n
+    for p, logical in enumerate(order):
+        phys[logical] = p
+    if _confinable(instrs, tile_bits, lambda q: phys[q]) <= _confinable(instrs, tile_bits, lambda q: q):
+        return instrs, None
+    out = []
+    for ins in instrs:
+        if isinstance(ins, GateInstance):
+            out.append(GateInstance(ins.gate, [phys[q] for q in ins.qubits], [phys[q] for q in ins.controls]))
+        elif isinstance(ins, DiagonalPhase):
+            terms = []
+            for t in ins.terms:
+                conds = []
+                for c in t.conds:
+                    m = 0
+                    for q in range(64):
+                        if (int(c) >> q) & 1:
+                            m |= 1 << phys[q]
+                    conds.append(m)
+                terms.append(type(t)(tuple(conds), t.angle))
+            out.append(DiagonalPhase(ins.n_qubits, terms))
+        else:
+            out.append(ins)
+    return out, phys
 
 
 def fuse_1q_runs(instrs):
@@ -263,12 +394,28 @@ def tile_block(instrs, tile_bits: int = TILE_BITS):
 
 
 def cpu_pipeline(circuit, max_k: int = MAX_K, tile_bits: int = TILE_BITS):
-    """The rewritten instruction list (GateInstance, DiagonalPhase, Tiled)."""
+    """`final_state_cpu_pipeline` (crates/aleph-ir/src/passes/mod.rs:192-200):
+    RelabelQubits, CancelInversePairs, Fuse1qRuns, Fuse2q, FuseKq(4),
+    TileBlock(15).  Returns (instructions, perm): the rewritten list
+    (GateInstance, DiagonalPhase, Tiled) and the relabelling
+    (perm[logical] = physical, or None), which `apply` undoes at the end."""
     instrs = [i for i in circuit.instructions if isinstance(i, (GateInstance, DiagonalPhase))]
-    return tile_block(fuse_kq(fuse_2q(fuse_1q_runs(instrs)), max_k), tile_bits)
+    instrs, perm = relabel_qubits(instrs, circuit.num_qubits, tile_bits)
+    instrs = tile_block(fuse_kq(fuse_2q(fuse_1q_runs(cancel_inverse_pairs(instrs))), max_k), tile_bits)
+    return instrs, perm
 
 
-def apply(o: "orc.OracleSv", st, instrs, tile_bits: int = TILE_BITS) -> None:
+def apply(o: "orc.OracleSv", st, instrs, tile_bits: int = TILE_BITS, perm=None, permute_input: bool = True) -> None:
+    """Apply a rewritten list; `perm` (from cpu_pipeline) is undone afterwards,
+    as run_optimized_with_pipeline does (crates/aleph-backend/src/lib.rs:394-405).
+    The reference always starts from |0...0>, which every relabelling leaves
+    unchanged; for another input state (`permute_input`) the state is first
+    carried into the physical order (the un-permutation by perm's inverse)."""
+    if perm is not None and permute_input:
+        inv = [0] * len(perm)
+        for logical, physical in enumerate(perm):
+            inv[physical] = logical
+        o.unpermute_state(st, inv)
     for ins in instrs:
         if isinstance(ins, Tiled):
             o.apply_tiled_block(st, ins.gates, tile_bits)
@@ -276,3 +423,11 @@ def apply(o: "orc.OracleSv", st, instrs, tile_bits: int = TILE_BITS) -> None:
             o.apply_diagonal_phase(st, ins)
         else:
             o.apply_gate(st, ins)
+    if perm is not None:
+        o.unpermute_state(st, perm)
+
+
+def run_final_state(o: "orc.OracleSv", st, circuit, tile_bits: int = TILE_BITS) -> None:
+    """The whole CPU final-state path: pipeline, application, un-permutation."""
+    instrs, perm = cpu_pipeline(circuit, tile_bits=tile_bits)
+    apply(o, st, instrs, tile_bits, perm)

@@ -84,3 +84,22 @@ def test_state_may_outlive_its_context():
     st = b.allocate(6)
     b.close()
     st.free()
+
+
+def test_rust_binding_declares_every_entry_point():
+    """crates/aleph-hip/src/ffi.rs declares every function of the header with
+    the same parameter count (the crate cannot be compiled in this image, so
+    this is its check against the C ABI)."""
+    import re
+    from pathlib import Path
+    root = Path(__file__).resolve().parent.parent
+    header = re.sub(r"/\*.*?\*/", "", (root / "include" / "aleph_hip.h").read_text(), flags=re.S)
+    protos = dict((m.group(1), m.group(2)) for m in re.finditer(
+        r"(?:aleph_hip_status|void)\s+(aleph_hip_\w+)\s*\(([^;]*)\)\s*;", header))
+    ffi = (root / "crates" / "aleph-hip" / "src" / "ffi.rs").read_text()
+    decls = dict((m.group(1), m.group(2)) for m in re.finditer(r"pub fn (aleph_hip_\w+)\(([^)]*)\)", ffi))
+    assert set(protos) == set(decls)
+    for name, args in protos.items():
+        n_c = 0 if args.strip() in ("", "void") else args.count(",") + 1
+        n_rs = 0 if not decls[name].strip() else decls[name].count(",") + 1
+        assert n_c == n_rs, name

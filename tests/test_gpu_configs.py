@@ -36,7 +36,7 @@ def oracle_final(orc, circuit, psi0=None, cap=30):
     st = o.allocate(circuit.num_qubits)
     if psi0 is not None:
         st.load(psi0)
-    P.apply(o, st, P.cpu_pipeline(circuit))
+    P.run_final_state(o, st, circuit)
     return st.amplitudes()
 
 
