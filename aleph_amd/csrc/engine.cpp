@@ -742,18 +742,34 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
       return rm;
     };
     std::vector<PhaseTerm> pend_reg, pend_gen;
+    // A term whose mask has no chunk bit outside the register bits depends
+    // only on the chunk number: the same for every thread of a workgroup.
+    auto thread_dep = [&](const PhaseTerm& t) { return (t.mask_out & S) != 0; };
     auto flush_phase = [&]() {
       if (pend_reg.empty() && pend_gen.empty()) return;
       DevOp ph{};
       ph.type = OP_PHASE;
-      // sort by accumulator class; class boundaries go into the matrix slots
-      std::stable_sort(pend_reg.begin(), pend_reg.end(),
-                       [](const PhaseTerm& x, const PhaseTerm& y) { return x.cls < y.cls; });
+      // Sort by accumulator class, and within a class the workgroup-uniform
+      // terms first.  The matrix slots hold the class boundaries
+      // (bounds[0..kCount]) and then each class's first thread-dependent
+      // term (split[0..kCount-1]), all relative to term_off.
+      std::stable_sort(pend_reg.begin(), pend_reg.end(), [&](const PhaseTerm& x, const PhaseTerm& y) {
+        if (x.cls != y.cls) return x.cls < y.cls;
+        return !thread_dep(x) && thread_dep(y);
+      });
       const uint32_t n_classes = 1 + pc.G + pc.G * (pc.G - 1) / 2;
       uint32_t* bounds = reinterpret_cast<uint32_t*>(ph.m);
+      uint32_t* split = bounds + n_classes + 1;
+      static_assert(sizeof(DevOp::m) >= (2 * (1 + kMaxRegBits + kMaxRegBits * (kMaxRegBits - 1) / 2) + 1) * 4,
+                    "phase class tables fit the matrix slots");
       for (uint32_t k = 0, t = 0; k <= n_classes; k++) {
         while (t < pend_reg.size() && pend_reg[t].cls < k) t++;
         bounds[k] = t;
+      }
+      for (uint32_t k = 0; k < n_classes; k++) {
+        uint32_t t = bounds[k];
+        while (t < bounds[k + 1] && !thread_dep(pend_reg[t])) t++;
+        split[k] = t;
       }
       ph.term_off = (uint32_t)p.terms.size();
       ph.n_terms = (uint32_t)pend_reg.size();
@@ -812,6 +828,8 @@ void emit_fused_pass(Plan& p, const PlannerCtx& pc, uint64_t S, const std::vecto
             t.mask_out = mask & ~rphys;
             t.rmask = reg_mask(mask);
             t.angle = angle;
+            t.c = std::cos(angle);
+            t.s = std::sin(angle);
             bool general = false;
             t.cls = phase_class(t.rmask, pc.G, &general);
             (general ? pend_gen : pend_reg).push_back(t);

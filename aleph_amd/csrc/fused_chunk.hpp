@@ -306,6 +306,62 @@ __device__ __forceinline__ void op_phase(C2<T> (&a)[1 << G], const DevOp* __rest
   }
 }
 
+// ---- phase ops in generated pass kernels ----
+// The host orders each class's terms uniform-first (terms that test only
+// bits outside the chunk, the same for every thread of a workgroup) and
+// records the split (PhaseSlots::kCount + 1 class bounds, then kCount
+// splits, in the op's matrix slots).  Generated kernels compute each
+// class's uniform factor e^{i sum angles} once per workgroup (one lane per
+// class, into LDS), and per thread multiply in only the unit factors of the
+// thread-dependent terms; no per-thread sincos and no per-thread pass over
+// the uniform terms.
+
+// Uniform factor of class k of phase op `op` for the chunk whose
+// outside-chunk index bits are `base`.
+template <int G>
+__device__ __forceinline__ double2 phase_uniform(const DevOp* __restrict__ op, const PhaseTerm* __restrict__ terms,
+                                                 uint64_t base, uint32_t k) {
+  using L = PhaseSlots<G>;
+  const uint32_t* bounds = reinterpret_cast<const uint32_t*>(op->m);
+  const uint32_t* split = bounds + L::kCount + 1;
+  const uint32_t b = op->term_off + bounds[k], e = op->term_off + split[k];
+  double acc = 0.0;
+  for (uint32_t t = b; t < e; t++) {
+    const uint64_t mo = terms[t].mask_out;
+    acc += ((base & mo) == mo) ? terms[t].angle : 0.0;
+  }
+  double sn, cs;
+  sincos(acc, &sn, &cs);
+  return make_double2(cs, sn);
+}
+
+// Factor of one class for this thread: the workgroup's uniform factor `u`
+// times the unit factors of the class's COUNT thread-dependent terms, which
+// start REL terms after the op's first term.
+template <int REL, int COUNT>
+__device__ __forceinline__ double2 phase_factor(const DevOp* __restrict__ op, const PhaseTerm* __restrict__ terms,
+                                                uint64_t tgo, double2 u) {
+  const PhaseTerm* tp = terms + op->term_off + REL;
+#pragma unroll
+  for (int j = 0; j < COUNT; j++) {
+    const uint64_t mo = tp[j].mask_out;
+    const bool on = (tgo & mo) == mo;
+    const double c = on ? tp[j].c : 1.0, sn = on ? tp[j].s : 0.0;
+    const double r = u.x * c - u.y * sn;
+    u.y = u.x * sn + u.y * c;
+    u.x = r;
+  }
+  return u;
+}
+
+template <typename T>
+__device__ __forceinline__ void phase_mul(C2<T>& a, double2 f) {
+  const T fr = (T)f.x, fi = (T)f.y;
+  const T x = a.x, y = a.y;
+  a.x = x * fr - y * fi;
+  a.y = x * fi + y * fr;
+}
+
 template <typename T, int G, int B>
 __device__ __forceinline__ void dispatch_1q(C2<T> (&a)[1 << G], const DevOp* op, uint64_t tgo,
                                             const uint64_t (&rg)[G], uint32_t b) {

@@ -56,7 +56,8 @@ struct alignas(16) PhaseTerm {
   double angle;
   uint32_t cls;
   uint32_t rmask;
-  uint32_t pad[2];
+  double c, s;       // cos(angle), sin(angle): the term's unit factor
+  uint64_t pad;
 };
 
 struct alignas(16) StageDev {

@@ -32,7 +32,8 @@ class DevOp(C.Structure):
 
 
 class PhaseTerm(C.Structure):
-    _fields_ = [("mask_out", u64), ("angle", f64), ("cls", u32), ("rmask", u32), ("pad", u32 * 2)]
+    _fields_ = [("mask_out", u64), ("angle", f64), ("cls", u32), ("rmask", u32), ("c", f64), ("s", f64),
+                ("pad", u64)]
 
 
 class Step(C.Structure):

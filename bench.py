@@ -62,7 +62,10 @@ def parse():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--no-jit", action="store_true",
                     help="run fused passes with the compiled op-loop kernel instead of per-pass generated kernels")
-    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--traffic-child", action="store_true",
+                    help="only reset, execute one plan and take the norm (the process rocprofv3 profiles)")
+    ap.add_argument("--child-precision", type=int, default=64, choices=(32, 64))
+    ap.add_argument("--child-workload", default="brickwall", choices=("brickwall", "qft"))
     return ap.parse_args()
 
 
@@ -184,13 +187,15 @@ JIT_KERNEL = "aleph_pass_f64"           # per-pass generated kernels
 
 
 def traffic_child(args) -> None:
-    """Run under rocprofv3 --pmc: one |0..0> reset (k_init_basis writes the
-    whole state), one execution of the plan, one norm (k_reduce_branch reads
-    the whole state).  The two full-state kernels calibrate the counters."""
+    """Run under rocprofv3 (--pmc or --kernel-trace): one |0..0> reset
+    (k_init_basis writes the whole state), one execution of the plan, one
+    norm (k_reduce_branch reads the whole state).  The two full-state kernels
+    calibrate the counters.  --child-precision and --child-workload pick the
+    plan (FP64 brickwall by default)."""
     from aleph_amd import workloads as W
     from aleph_amd.backend import HipSvBackend
-    b = HipSvBackend(seed=1)
-    c = W.brickwall(args.n, args.depth, seed=args.seed)
+    b = HipSvBackend(seed=1, precision=args.child_precision)
+    c = W.qft(args.n) if args.child_workload == "qft" else W.brickwall(args.n, args.depth, seed=args.seed)
     plan = b.compile(c, local_qubits=args.local_qubits, max_ops=args.max_ops, jit=not args.no_jit)
     st = b.allocate(args.n)
     plan.execute(st)

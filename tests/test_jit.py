@@ -18,7 +18,13 @@ from tests.test_oracle import random_circuit
 HIPRTC = Path("/opt/rocm/lib/libhiprtc.so")
 CSRC = Path(__file__).resolve().parent.parent / "aleph_amd" / "csrc"
 HEADERS = ["device_types.hpp", "plan.hpp", "fused_chunk.hpp"]
-OP_CALL = {1: "op_1q<", 2: "op_2q<", 3: "op_diag<", 4: "op_sign<", 5: "op_phase<"}
+OP_CALL = {1: "op_1q<", 2: "op_2q<", 3: "op_diag<", 4: "op_sign<", 5: "phase"}
+# A phase op is written out as a block of per-class factors (phase_factor,
+# from the workgroup's uniform table) and slot multiplies, or, with terms on
+# three or more register bits, as a call of the generic op_phase.
+# A 4x4 op whose matrix is a unit permutation (SWAP, CNOT) renames register
+# slots instead ("// slot renaming").
+OP_RE = r"(op_\w+<|const DevOp\* po|// slot renaming)"
 
 
 def _compile(src: str) -> tuple[int, str]:
@@ -45,6 +51,10 @@ def _plans():
         HostPlan(W.hea(14, 3, seed=1), precision=32),
         HostPlan(random_circuit(12, 60, rng, kq=False), precision=64, local_qubits=10),
         HostPlan(random_circuit(13, 60, rng, kq=False), precision=32, reg_bits=5, nontemporal=True),
+        HostPlan(W.qft(16), precision=64),
+        HostPlan(W.qft(13), precision=64, fuse_2q=False),
+        HostPlan(W.qft(14), precision=32, reg_bits=5),
+        HostPlan(W.qaoa_maxcut(14, W.random_3_regular(14, 5), [0.4, 1.1], [0.7, 0.2]), precision=64),
     ]
 
 
@@ -54,8 +64,13 @@ def test_sources_list_the_pass_ops_in_order():
             src = hp.jit_sources[i]
             stages = hp.stages[p.stage_begin:p.stage_begin + p.n_stages]
             want = [OP_CALL[hp.ops[o].type] for sd in stages for o in range(sd.op_begin, sd.op_end)]
-            got = re.findall(r"(op_\w+<)", src)
+            alias = {"op_phase<": "phase", "const DevOp* po": "phase", "// slot renaming": "op_2q<"}
+            got = [alias.get(g, g) for g in re.findall(OP_RE, src)]
             assert got == want
+            n_fast = sum(1 for sd in stages for o in range(sd.op_begin, sd.op_end)
+                         if hp.ops[o].type == 5 and hp.ops[o].b == 0)
+            assert src.count("const DevOp* po") == n_fast
+            assert (n_fast > 0) == ("phase_uniform<" in src)
             assert src.count("io.load(a)") == 1 and src.count("io.store(a)") == 1
             if hp.precision == 64 and p.n_stages > 1:
                 # FP64: real and imaginary parts exchanged in turn through half the LDS
@@ -74,11 +89,12 @@ def test_generated_sources_compile_for_gfx950():
     seen = set()
     for hp in _plans():
         for src in hp.jit_sources:
-            if src in seen or "op_phase<" in src:
+            if src in seen:
                 continue
             seen.add(src)
-            if len(seen) > 6:
+            if len(seen) > 9:
                 break
             rc, log = _compile(src)
             assert rc == 0, log
     assert len(seen) >= 4
+    assert any("phase_uniform<" in s for s in seen)
