@@ -636,23 +636,39 @@ std::unique_ptr<Plan> plan_learned(aleph_hip_ctx* c, const std::vector<ProgInstr
   const auto t0 = std::chrono::steady_clock::now();
   struct Cand {
     uint32_t m, budget;
-    bool f2q;
+    bool f2q, nt;
+    uint32_t kq;
     std::unique_ptr<Plan> p;
     double us = 0.0;
   };
+  // Candidates: chunk qubits x pass budget x 4x4 pair fusion x non-temporal
+  // streaming, plus dense-block plans (k = 4, 5 on the matrix cores) at the
+  // rule planner's chunking.  The code variant (generated or compiled pass
+  // kernels) is the caller's (base.jit) and is a model feature.
   std::vector<Cand> cands;
   for (uint32_t m : {10u, 11u, 12u}) {
     if (m > n && m != 10u) continue;
     for (uint32_t budget : {12u, 24u, 48u}) {
       for (bool f2q : {true, false}) {
-        PlanOptions po = base;
-        po.planner = ALEPH_HIP_PLANNER_RULE;
-        po.local_qubits = std::min(m, n);
-        po.max_ops = budget;
-        po.fuse_2q = f2q;
-        cands.push_back(Cand{m, budget, f2q, build_plan(prog, n, precision, po)});
+        for (bool nt : {false, true}) {
+          PlanOptions po = base;
+          po.planner = ALEPH_HIP_PLANNER_RULE;
+          po.local_qubits = std::min(m, n);
+          po.max_ops = budget;
+          po.fuse_2q = f2q;
+          po.nontemporal = nt;
+          po.kq_width = 0;
+          cands.push_back(Cand{m, budget, f2q, nt, 0u, build_plan(prog, n, precision, po)});
+        }
       }
     }
+  }
+  for (uint32_t kq : {4u, 5u}) {
+    if (n < kq + 4) continue;
+    PlanOptions po = base;
+    po.planner = ALEPH_HIP_PLANNER_RULE;
+    po.kq_width = kq;
+    cands.push_back(Cand{std::min<uint32_t>(12u, n), 24u, true, false, kq, build_plan(prog, n, precision, po)});
   }
   std::vector<float> rows;
   std::vector<uint32_t> owner;
@@ -684,7 +700,9 @@ std::unique_ptr<Plan> plan_learned(aleph_hip_ctx* c, const std::vector<ProgInstr
   st.planner_used = ALEPH_HIP_PLANNER_LEARNED;
   st.chosen_local_qubits = std::min(cands[best].m, n);
   st.chosen_max_ops = cands[best].budget;
-  st.chosen_flags = cands[best].f2q ? 0u : (uint32_t)ALEPH_HIP_PLAN_NO_FUSE_2Q;
+  st.chosen_flags = (cands[best].f2q ? 0u : (uint32_t)ALEPH_HIP_PLAN_NO_FUSE_2Q) |
+                    (cands[best].nt ? (uint32_t)ALEPH_HIP_PLAN_NONTEMPORAL : 0u);
+  st.chosen_kq_width = cands[best].kq;
   st.n_candidates = (uint32_t)cands.size();
   st.predicted_us = cands[best].us;
   st.plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -1213,10 +1213,11 @@ void plan_features(const Plan& p, std::vector<float>& rows, std::vector<uint32_t
         if (op.type != OP_SIGN && op.type != OP_PHASE && op.cmask != 0) nctl += 1;
       }
     }
-    // Slot 0 stays 0: the model predicts a launch on 2^kCostRefQubits
-    // amplitudes and the planner scales by the state size, so predictions
-    // hold at qubit counts outside the training corpus.
-    const float f[kPlanFeatures] = {0.f,
+    // The qubit count is not a feature: the model predicts a launch on
+    // 2^kCostRefQubits amplitudes and the planner scales by the state size,
+    // so predictions hold at qubit counts outside the training corpus.
+    // Slot 0 is the code variant (1: kernel generated for the pass).
+    const float f[kPlanFeatures] = {p.opts.jit ? 1.f : 0.f,
                                     p.precision == ALEPH_HIP_F64 ? 1.f : 0.f,
                                     (float)pd.m / 13.f,
                                     (float)pd.n_stages / 8.f,
@@ -1237,12 +1238,17 @@ void plan_features(const Plan& p, std::vector<float>& rows, std::vector<uint32_t
   }
 }
 
+// Standalone steps are priced from their measured rates on MI355X: a dense
+// block of k <= 5 qubits or a phase polynomial streams at about 4.8 TB/s,
+// an FP64 k = 6 block is matrix-core bound at about twice that time.
 double plan_standalone_us(const Plan& p) {
   const double bpa = p.precision == ALEPH_HIP_F64 ? 16.0 : 8.0;
-  const double sweep_us = 2.0 * std::ldexp(bpa, (int)p.n) / 4e12 * 1e6;
+  const double sweep_us = 2.0 * std::ldexp(bpa, (int)p.n) / 4.8e12 * 1e6;
   double us = 0.0;
-  for (const Step& s : p.steps)
-    if (s.kind != Step::FUSED) us += sweep_us;
+  for (const Step& s : p.steps) {
+    if (s.kind == Step::FUSED) continue;
+    us += (s.kind == Step::KQ && s.k >= 6 && p.precision == ALEPH_HIP_F64) ? 2.0 * sweep_us : sweep_us;
+  }
   return us;
 }
 

@@ -166,6 +166,72 @@ __device__ __forceinline__ void op_2q(C2<T> (&a)[1 << G], const DevOp* __restric
   quartets_2q<T, G, A, B, 0, true>(a, op, on);
 }
 
+// FP32 4x4 op on the matrix cores: v_mfma_f32_4x4x1f32 (16 blocks of
+// 4 lanes).  In block b, lane 4b + n supplies A[n] and B[n] and receives
+// column n of C += A B^T, i.e. C[i] += A(lane 4b + i) * B(own).  With each
+// lane supplying row n of the op's matrix column k as A and its own
+// quartet's element k as B, four K steps leave (M x)[i] in the lane's C[i]:
+// every lane's quartet is its own column and nothing moves between lanes.
+// Real and imaginary parts: yr = Mr xr - Mi xi, yi = Mi xr + Mr xi, eight
+// K steps each, so 16 MFMAs per quartet replace 64 VALU FMAs, on the
+// matrix pipe beside the vector ALU.  The lane's matrix row (8 floats of
+// the plan's FP32 op matrix) is read with two 16-byte loads.
+typedef float v4f32 __attribute__((ext_vector_type(4)));
+
+// All quartets of the op advance together through the K steps, so the
+// matrix pipe always has 2 * 2^(G-2) independent accumulation chains.
+template <int G, int A, int B>
+__device__ __forceinline__ void quartets_2q_mfma(C2<float> (&a)[1 << G], const float (&re)[4], const float (&im)[4],
+                                                 const float (&nim)[4]) {
+  constexpr int NQ = 1 << (G - 2);
+  v4f32 cr[NQ], ci[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    cr[q] = v4f32{0.f, 0.f, 0.f, 0.f};
+    ci[q] = v4f32{0.f, 0.f, 0.f, 0.f};
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const int r = ins0(ins0(q, B), A);
+      const int slot = r | ((k & 1) << B) | ((k >> 1) << A);
+      cr[q] = __builtin_amdgcn_mfma_f32_4x4x1f32(re[k], a[slot].x, cr[q], 0, 0, 0);
+      ci[q] = __builtin_amdgcn_mfma_f32_4x4x1f32(im[k], a[slot].x, ci[q], 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const int r = ins0(ins0(q, B), A);
+      const int slot = r | ((k & 1) << B) | ((k >> 1) << A);
+      cr[q] = __builtin_amdgcn_mfma_f32_4x4x1f32(nim[k], a[slot].y, cr[q], 0, 0, 0);
+      ci[q] = __builtin_amdgcn_mfma_f32_4x4x1f32(re[k], a[slot].y, ci[q], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int r = ins0(ins0(q, B), A);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int slot = r | ((i & 1) << B) | ((i >> 1) << A);
+      a[slot].x = cr[q][i];
+      a[slot].y = ci[q][i];
+    }
+  }
+}
+
+// A = matrix MSB register bit, B = LSB, A > B; uncontrolled ops only.
+template <int G, int A, int B>
+__device__ __forceinline__ void op_2q_mfma(C2<float> (&a)[1 << G], const DevOp* __restrict__ op) {
+  const uint32_t n = threadIdx.x & 3u;          // this lane's matrix row within its 4-lane block
+  const float* row = reinterpret_cast<const float*>(op->m) + 8 * n;
+  const v4f32 lo = *reinterpret_cast<const v4f32*>(row);       // M[n][0], M[n][1] (re, im interleaved)
+  const v4f32 hi = *reinterpret_cast<const v4f32*>(row + 4);   // M[n][2], M[n][3]
+  const float re[4] = {lo[0], lo[2], hi[0], hi[2]};
+  const float im[4] = {lo[1], lo[3], hi[1], hi[3]};
+  const float nim[4] = {-lo[1], -lo[3], -hi[1], -hi[3]};
+  quartets_2q_mfma<G, A, B>(a, re, im, nim);
+}
+
 template <typename T, int G>
 __device__ __forceinline__ void op_diag(C2<T> (&a)[1 << G], const DevOp* __restrict__ op, uint64_t tgo,
                                         const uint64_t (&rg)[G]) {
@@ -310,26 +376,32 @@ __device__ __forceinline__ void op_phase(C2<T> (&a)[1 << G], const DevOp* __rest
 // The host orders each class's terms uniform-first (terms that test only
 // bits outside the chunk, the same for every thread of a workgroup) and
 // records the split (PhaseSlots::kCount + 1 class bounds, then kCount
-// splits, in the op's matrix slots).  Generated kernels compute each
-// class's uniform factor e^{i sum angles} once per workgroup (one lane per
-// class, into LDS), and per thread multiply in only the unit factors of the
-// thread-dependent terms; no per-thread sincos and no per-thread pass over
-// the uniform terms.
+// splits, in the op's matrix slots).  Generated kernels evaluate every term
+// of the pass's phase ops for the chunk once per workgroup (one lane per
+// term, into LDS), sum each class's uniform terms and take e^{i sum} (one
+// lane per class, into LDS), and per thread multiply in only the unit
+// factors of the thread-dependent terms: no per-thread sincos and no
+// per-thread pass over the uniform terms.
 
-// Uniform factor of class k of phase op `op` for the chunk whose
-// outside-chunk index bits are `base`.
+// Contribution of the op's term `rel` (relative to term_off) for the chunk
+// whose outside-chunk index bits are `base`: its angle if it fires.  Only
+// the workgroup-uniform terms' values are used (phase_class_uniform).
+__device__ __forceinline__ double phase_term_uniform(const DevOp* __restrict__ op,
+                                                     const PhaseTerm* __restrict__ terms, uint64_t base,
+                                                     uint32_t rel) {
+  const PhaseTerm& pt = terms[op->term_off + rel];
+  return ((base & pt.mask_out) == pt.mask_out) ? pt.angle : 0.0;
+}
+
+// Uniform factor of class k: e^{i sum} over the class's uniform terms,
+// whose per-chunk angles `ua` (indexed relative to term_off) are in LDS.
 template <int G>
-__device__ __forceinline__ double2 phase_uniform(const DevOp* __restrict__ op, const PhaseTerm* __restrict__ terms,
-                                                 uint64_t base, uint32_t k) {
+__device__ __forceinline__ double2 phase_class_uniform(const DevOp* __restrict__ op, const double* ua, uint32_t k) {
   using L = PhaseSlots<G>;
   const uint32_t* bounds = reinterpret_cast<const uint32_t*>(op->m);
   const uint32_t* split = bounds + L::kCount + 1;
-  const uint32_t b = op->term_off + bounds[k], e = op->term_off + split[k];
   double acc = 0.0;
-  for (uint32_t t = b; t < e; t++) {
-    const uint64_t mo = terms[t].mask_out;
-    acc += ((base & mo) == mo) ? terms[t].angle : 0.0;
-  }
+  for (uint32_t t = bounds[k]; t < split[k]; t++) acc += ua[t];
   double sn, cs;
   sincos(acc, &sn, &cs);
   return make_double2(cs, sn);

@@ -145,6 +145,8 @@ pub struct aleph_hip_plan_stats {
     pub chosen_flags: u32,
     pub n_candidates: u32,
     pub predicted_us: f64,
+    pub chosen_kq_width: u32,
+    pub reserved: u32,
 }
 
 #[repr(C)]

@@ -201,6 +201,8 @@ typedef struct aleph_hip_plan_stats {
   /* LEARNED: the candidate options the cost model chose and its prediction */
   uint32_t chosen_local_qubits, chosen_max_ops, chosen_flags, n_candidates;
   double predicted_us;
+  uint32_t chosen_kq_width;  /* LEARNED: dense-block width of the chosen candidate (0 = LDS passes) */
+  uint32_t reserved;
 } aleph_hip_plan_stats;
 
 aleph_hip_status aleph_hip_plan_create(aleph_hip_ctx* ctx, uint32_t num_qubits, uint32_t precision,

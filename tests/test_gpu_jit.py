@@ -58,3 +58,21 @@ def test_generated_kernels_equal_compiled_kernels(precision, tol, name, seed, ma
     st.load(psi)
     o.run(c, st)
     assert max_abs(jit, st.amplitudes()) < tol, f"{name} seed={seed}"
+
+
+def test_fp32_matrix_core_4x4_ops_equal_vector_ops(monkeypatch):
+    """ALEPH_HIP_FP32_MFMA=1: FP32 4x4 ops of generated kernels on the matrix
+    cores (v_mfma_f32_4x4x1f32, each lane's quartet its own column) give the
+    vector-ALU result to FP32 rounding and the oracle to 1e-5."""
+    c = W.brickwall(16, 8, seed=21)
+    psi = W.gaussian_state(16, 3)
+    b = HipSvBackend(seed=3, precision=32)
+    ref = _run(b, c, psi, jit=True)
+    monkeypatch.setenv("ALEPH_HIP_FP32_MFMA", "1")
+    mf = _run(b, c, psi, jit=True)
+    assert max_abs(mf, ref) <= 1e-6
+    o = OracleSv(cap=30)
+    st = o.allocate(16)
+    st.load(psi)
+    o.run(c, st)
+    assert max_abs(mf, st.amplitudes()) < 1e-5

@@ -25,6 +25,7 @@ OP_CALL = {1: "op_1q<", 2: "op_2q<", 3: "op_diag<", 4: "op_sign<", 5: "phase"}
 # A 4x4 op whose matrix is a unit permutation (SWAP, CNOT) renames register
 # slots instead ("// slot renaming").
 OP_RE = r"(op_\w+<|const DevOp\* po|// slot renaming)"
+# With ALEPH_HIP_FP32_MFMA=1, FP32 uncontrolled 4x4 ops run on the matrix cores (op_2q_mfma).
 
 
 def _compile(src: str) -> tuple[int, str]:
@@ -64,19 +65,22 @@ def test_sources_list_the_pass_ops_in_order():
             src = hp.jit_sources[i]
             stages = hp.stages[p.stage_begin:p.stage_begin + p.n_stages]
             want = [OP_CALL[hp.ops[o].type] for sd in stages for o in range(sd.op_begin, sd.op_end)]
-            alias = {"op_phase<": "phase", "const DevOp* po": "phase", "// slot renaming": "op_2q<"}
+            alias = {"op_phase<": "phase", "const DevOp* po": "phase", "// slot renaming": "op_2q<",
+                     "op_2q_mfma<": "op_2q<"}
             got = [alias.get(g, g) for g in re.findall(OP_RE, src)]
             assert got == want
             n_fast = sum(1 for sd in stages for o in range(sd.op_begin, sd.op_end)
                          if hp.ops[o].type == 5 and hp.ops[o].b == 0)
             assert src.count("const DevOp* po") == n_fast
-            assert (n_fast > 0) == ("phase_uniform<" in src)
+            assert (n_fast > 0) == ("phase_class_uniform<" in src)
             assert src.count("io.load(a)") == 1 and src.count("io.store(a)") == 1
             if hp.precision == 64 and p.n_stages > 1:
                 # FP64: real and imaginary parts exchanged in turn through half the LDS
                 assert src.count("io.exchange_split(a, ") == p.n_stages - 1
                 assert src.count("io.enter(") == 1 and "io.put(" not in src
-                assert f"__launch_bounds__({1 << (p.m - p.reg_bits)}, 4)" in src
+                # four waves per SIMD (128 registers), two for passes with phase ops
+                waves = 2 if n_fast else 4
+                assert f"__launch_bounds__({1 << (p.m - p.reg_bits)}, {waves})" in src
             else:
                 assert src.count("io.enter(") == p.n_stages
                 assert src.count("io.put(a)") == p.n_stages - 1 == src.count("io.get(a)")
@@ -97,4 +101,4 @@ def test_generated_sources_compile_for_gfx950():
             rc, log = _compile(src)
             assert rc == 0, log
     assert len(seen) >= 4
-    assert any("phase_uniform<" in s for s in seen)
+    assert any("phase_class_uniform<" in s for s in seen)
