@@ -65,7 +65,7 @@ class PlanStats(C.Structure):
     _fields_ = [("plan_ms", f64), ("input_gates", u32), ("n_passes", u32), ("n_stages", u32), ("n_ops", u32),
                 ("n_launches", u32), ("planner_used", u32), ("sweep_bytes", f64), ("chosen_local_qubits", u32),
                 ("chosen_max_ops", u32), ("chosen_flags", u32), ("n_candidates", u32), ("predicted_us", f64),
-                ("chosen_kq_width", u32), ("reserved", u32)]
+                ("chosen_kq_width", u32), ("chosen_rule", u32), ("rule_predicted_us", f64)]
 
 
 MODEL_INPUTS = 16
@@ -103,6 +103,14 @@ class BatchView(C.Structure):
                 ("terms", P), ("n_terms", u32), ("term_bytes", u32),
                 ("offsets", PU64), ("n_instances", u32), ("n_rounds", u32),
                 ("round_begin", PU32), ("threads", u32), ("lds_bytes", u32), ("total_amps", u64)]
+
+
+CHANNEL_PAULI, CHANNEL_KRAUS = 0, 1
+
+
+class Channel(C.Structure):
+    _fields_ = [("after", u32), ("kind", u32), ("n_qubits", u32), ("n_branches", u32), ("qubits", u32 * 2),
+                ("weights", PF64), ("paulis", PU8), ("kraus", PF64)]
 
 
 # name -> (restype, argtypes); every symbol include/aleph_hip.h declares.
@@ -178,6 +186,11 @@ SIGNATURES = {
     "aleph_hip_batch_build_host_qaoa": (i32, [u32, PU32, PU64, PU32, u32, PF64, PF64, u32, C.POINTER(BatchOpts),
                                               C.POINTER(P), C.POINTER(Error)]),
     "aleph_hip_batch_view_get": (i32, [P, C.POINTER(BatchView)]),
+    "aleph_hip_dist_alloc_rank": (i32, [P, u32, u32, u32, C.POINTER(P)]),
+    "aleph_hip_gate_matrix": (i32, [C.POINTER(Gate), u32, PF64, PU32]),
+    "aleph_hip_dist_exchange": (i32, [P, C.POINTER(P), u32, PU32, u32]),
+    "aleph_hip_run_noisy": (i32, [P, u32, PU32, C.c_size_t, PF64, C.c_size_t, C.POINTER(Channel), u32, PF64, u64,
+                                  u32, u64, u64, C.c_size_t, PU64]),
 }
 
 _lib = None

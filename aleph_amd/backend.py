@@ -383,3 +383,42 @@ def run_with_outcomes(backend: HipSvBackend, circuit: Circuit):
 
 def run(backend: HipSvBackend, circuit: Circuit) -> HipState:
     return run_with_outcomes(backend, circuit)[0]
+
+
+def run_circuit_with_outcomes(backend: HipSvBackend, circuit: Circuit, uniforms=None, **plan_kw):
+    """Whole-circuit path: every measurement-free segment is planned into
+    fused passes and executed in one library call; measurements in between
+    go through `measure` (or `measure_with_uniform` with the next of
+    `uniforms`, consumed only when the outcome is not forced), with the
+    records of run_with_outcomes (crates/aleph-backend/src/lib.rs:211-248)."""
+    if circuit.num_qubits == 0 and len(circuit) == 0:
+        raise EmptyCircuit("cannot run an empty circuit")
+    state = backend.allocate(circuit.num_qubits)
+    outcomes = []
+    draws = iter(uniforms) if uniforms is not None else None
+    segment = Circuit(circuit.num_qubits)
+
+    def flush():
+        nonlocal segment
+        if len(segment):
+            backend.run_program(state, segment, **plan_kw)
+        segment = Circuit(circuit.num_qubits)
+
+    for idx, inst in enumerate(circuit.instructions):
+        if isinstance(inst, tuple) and inst and inst[0] == "measure":
+            flush()
+            _, q, clbit = inst
+            if draws is None:
+                bit = backend.measure(state, q)
+            else:
+                u = next(draws)
+                bit, used = backend.measure_with_uniform(state, q, u)
+                if not used:      # forced outcome: the uniform stays for the next measurement
+                    draws = iter([u, *draws])
+            outcomes.append(MeasurementRecord(idx, q, clbit, bit))
+        elif isinstance(inst, (GateInstance, DiagonalPhase, Barrier)):
+            segment.add(inst)
+        else:
+            raise UnsupportedInstruction(f"unsupported instruction {inst!r}", kind=type(inst).__name__)
+    flush()
+    return state, outcomes

@@ -22,7 +22,7 @@ LIB = PKG / "libaleph_hip.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["capi.cpp", "engine.cpp", "batch.cpp", "jit.cpp", "k_gates.hip", "k_fused.hip", "k_kq_mfma.hip", "k_batch.hip", "k_mlp.hip", "k_readout.hip"]
+SOURCES = ["capi.cpp", "engine.cpp", "batch.cpp", "jit.cpp", "k_gates.hip", "k_fused.hip", "k_kq_mfma.hip", "k_batch.hip", "k_mlp.hip", "k_readout.hip", "noise.cpp", "k_noise.hip", "k_exchange.hip"]
 
 COMMON = [
     "-O3",

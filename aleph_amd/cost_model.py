@@ -2,16 +2,24 @@
 
 A perceptron (16 pass features -> 3 x 64 ReLU -> 1) predicts ln(µs) of
 one fused-pass launch, scaled to a 2^24-amplitude state (the planner
-multiplies by 2^(n-24), so the model carries to qubit counts it was not
-trained at).  Training data are the launch times the engine
-records for candidate plans (`aleph_hip_plan_execute_timed`) paired with
-the features of the same launches (`aleph_hip_plan_features`); training,
-inference and the gradient run as the library's own HIP kernels.  With a
-model set on the backend, `compile(..., planner=PLANNER_LEARNED)` plans
-every candidate (chunk qubits x pass budget x 4x4 fusion) and keeps the
-one with the smallest predicted time; without a model the rule planner
-plans.  `python -m aleph_amd.cost_model --out PATH` collects timings on
-the device, trains, evaluates on held-out circuits and writes the model.
+multiplies by 2^(n-24)); one feature is the state's size, since a state
+that stays in the 256 MiB Infinity Cache between passes streams faster.
+Training data are the launch times the engine records for candidate plans
+(`aleph_hip_plan_execute_timed`) paired with the features of the same
+launches (`aleph_hip_plan_features`); training, inference and the gradient
+run as the library's own HIP kernels.  With a model set on the backend,
+`compile(..., planner=PLANNER_LEARNED)` plans every candidate (chunk qubits
+x pass budget x 4x4 fusion x non-temporal streaming, dense blocks of width
+3..6, generated or compiled pass kernels) and keeps the one with the
+smallest predicted time when it beats the rule planner's plan by at least
+1 %; without a model the rule planner plans.
+
+    python -m aleph_amd.cost_model collect --slice 0:75 --data a.npz
+    python -m aleph_amd.cost_model collect --slice 75:150 --data b.npz
+    python -m aleph_amd.cost_model train --data a.npz b.npz --out cost_model.json
+
+times a seeded sample of candidates of each training circuit on the
+device, trains, evaluates on the held-out circuits and writes the model.
 """
 
 from __future__ import annotations
@@ -32,8 +40,8 @@ from .backend import HipSvBackend, _ptr
 
 DEFAULT_PATH = Path(__file__).resolve().parent / "cost_model.json"
 REF_QUBITS = 24   # launch times are learned per 2^24 amplitudes (kCostRefQubits in the library)
-CANDIDATES = [dict(local_qubits=m, max_ops=b, fuse_2q=f) for m in (10, 11, 12) for b in (12, 24, 48)
-              for f in (True, False)]
+CANDIDATES = [dict(local_qubits=m, max_ops=b, fuse_2q=f, nontemporal=nt) for m in (10, 11, 12)
+              for b in (12, 24, 48) for f in (True, False) for nt in (False, True)]
 
 
 class CostModel:
@@ -140,30 +148,61 @@ def record(backend: HipSvBackend, circuit, state, reps: int = 2, **plan_kw):
     return X, y, float(best.sum())
 
 
-def collect(backends, specs, candidates=CANDIDATES, reps: int = 2, log=print, jit: bool = False):
-    """Timings of every candidate plan of every circuit; circuit k runs on
-    backends[k % len(backends)] (e.g. alternating FP64 and FP32).  With
-    `jit`, fused passes run as per-pass generated kernels."""
-    Xs, ys = [], []
+RULE = dict(local_qubits=12, max_ops=24, fuse_2q=True, nontemporal=False)
+
+
+def sample_candidates(rng: np.random.Generator, generated: int = 7, compiled: int = 3):
+    """The candidates timed for one corpus circuit: the rule planner's own
+    options plus a seeded sample of the others, as generated kernels, and a
+    few (the rule's among them) as the compiled op loop, so the model sees
+    both code variants the learned planner chooses between."""
+    others = [c for c in CANDIDATES if c != RULE]
+    pick = [others[i] for i in rng.choice(len(others), size=generated - 1, replace=False)]
+    out = [dict(RULE, jit=True)] + [dict(c, jit=True) for c in pick]
+    comp = [RULE] + [others[i] for i in rng.choice(len(others), size=compiled - 1, replace=False)]
+    return out + [dict(c, jit=False) for c in comp]
+
+
+def collect(backends, specs, reps: int = 2, log=print, seed: int = 0, checkpoint=None, deadline=None,
+            generated: int = 7, compiled: int = 3):
+    """Timings of a seeded sample of candidate plans (sample_candidates) of
+    every circuit; circuit k runs on backends[k % len(backends)] (e.g.
+    alternating FP64 and FP32).  With `checkpoint`, the rows so far are
+    saved after every circuit; collection stops before `deadline`
+    (time.time() value) and returns what it has."""
+    Xs, ys, done = [], [], 0
     states = {}
+    rng = np.random.default_rng(seed)
     for k, spec in enumerate(specs):
+        if deadline is not None and time.time() > deadline:
+            log(f"collect: deadline reached after {done} circuits")
+            break
         backend = backends[k % len(backends)]
         c = W.corpus_circuit(spec)
         key = (id(backend), c.num_qubits)
         if key not in states:
-            states[key] = backend.allocate(c.num_qubits)
-        for cand in candidates:
-            X, y, _ = record(backend, c, states[key], reps=reps, jit=jit, **cand)
+            for s in states.values():
+                s.free()
+            states = {key: backend.allocate(c.num_qubits)}
+        t0 = time.time()
+        for cand in sample_candidates(rng, generated, compiled):
+            X, y, _ = record(backend, c, states[key], reps=reps, **cand)
             Xs.append(X)
             ys.append(y)
+        done += 1
+        if checkpoint:
+            np.savez(checkpoint, X=np.concatenate(Xs), y=np.concatenate(ys), circuits=done)
         if log:
-            log(f"collect: {k + 1}/{len(specs)} circuits, {sum(len(v) for v in ys)} launches")
+            log(f"collect: {k + 1}/{len(specs)} circuits ({spec[0]} n={spec[1]} p{backend.precision}, "
+                f"{time.time() - t0:.1f} s), {sum(len(v) for v in ys)} launches")
     for st in states.values():
         st.free()
+    if not ys:
+        return np.zeros((0, L.MODEL_INPUTS), np.float32), np.zeros(0, np.float32)
     return np.concatenate(Xs), np.concatenate(ys)
 
 
-def evaluate(backends, models, specs, reps: int = 2, log=print, jit: bool = False):
+def evaluate(backends, models, specs, reps: int = 2, log=print, jit: bool = True):
     """Measured ms of the rule plan and of the learned plan per held-out
     circuit (circuit k on backends[k % len(backends)] with models[...])."""
     for m in models:
@@ -175,7 +214,9 @@ def evaluate(backends, models, specs, reps: int = 2, log=print, jit: bool = Fals
         c = W.corpus_circuit(spec)
         key = (id(backend), c.num_qubits)
         if key not in states:
-            states[key] = backend.allocate(c.num_qubits)
+            for s in states.values():
+                s.free()
+            states = {key: backend.allocate(c.num_qubits)}
         _, _, rule_ms = record(backend, c, states[key], reps=reps, jit=jit)
         plan = backend.compile(c, planner=L.PLANNER_LEARNED, jit=jit)
         st = plan.stats()
@@ -186,53 +227,88 @@ def evaluate(backends, models, specs, reps: int = 2, log=print, jit: bool = Fals
         plan.free()
         rows.append({"spec": list(spec), "precision": backend.precision, "rule_ms": rule_ms,
                      "learned_ms": float(best),
-                     "chosen": [st["chosen_local_qubits"], st["chosen_max_ops"], st["chosen_flags"]],
-                     "predicted_ms": st["predicted_us"] / 1e3, "plan_ms": st["plan_ms"]})
+                     "chosen": [st["chosen_local_qubits"], st["chosen_max_ops"], st["chosen_flags"],
+                                st["chosen_kq_width"]],
+                     "kept_rule": bool(st["chosen_rule"]),
+                     "predicted_ms": st["predicted_us"] / 1e3, "rule_predicted_ms": st["rule_predicted_us"] / 1e3,
+                     "plan_ms": st["plan_ms"]})
         if log:
-            log(f"eval {spec} p{backend.precision}: rule {rule_ms:.2f} ms, learned {best:.2f} ms, "
-                f"chose {rows[-1]['chosen']}")
+            log(f"eval {k + 1}/{len(specs)} {spec} p{backend.precision}: rule {rule_ms:.2f} ms, "
+                f"learned {best:.2f} ms, chose {rows[-1]['chosen']}")
     for s in states.values():
         s.free()
     return rows
 
 
+def corpus_split(train: int, held_out: int, seed: int, n_min: int, n_max: int, d_max: int):
+    specs = W.planner_corpus(train + held_out, seed, n_min=n_min, n_max=n_max, d_max=d_max)
+    return specs[:train], specs[train:]
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description="train the learned planner's cost model on this device")
+    ap.add_argument("command", choices=["collect", "train", "all"],
+                    help="collect: time candidate plans of a slice of the training corpus into --data; "
+                         "train: train on --data files, evaluate on the held-out corpus, write --out; "
+                         "all: both in one process")
     ap.add_argument("--out", default=str(DEFAULT_PATH))
-    ap.add_argument("--train", type=int, default=80, help="training circuits (alternating FP64 / FP32)")
-    ap.add_argument("--held-out", type=int, default=16, help="evaluation circuits")
-    ap.add_argument("--n-min", type=int, default=22)
-    ap.add_argument("--n-max", type=int, default=25)
+    ap.add_argument("--data", nargs="*", default=[], help="collected .npz files (collect writes the first)")
+    ap.add_argument("--train", type=int, default=150, help="training circuits (alternating FP64 / FP32)")
+    ap.add_argument("--held-out", type=int, default=40, help="evaluation circuits")
+    ap.add_argument("--slice", default="0:", help="collect only training circuits [a:b] of the corpus")
+    ap.add_argument("--n-min", type=int, default=24)
+    ap.add_argument("--n-max", type=int, default=30)
     ap.add_argument("--d-max", type=int, default=100)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=80)
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--jit", action="store_true", help="time per-pass generated kernels (the default code path)")
+    ap.add_argument("--minutes", type=float, default=0.0, help="stop collecting after this long (0: no limit)")
+    ap.add_argument("--generated", type=int, default=3, help="candidates timed per circuit as generated kernels")
+    ap.add_argument("--compiled", type=int, default=3, help="candidates timed per circuit as the compiled op loop")
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
     backends = [HipSvBackend(seed=1, precision=64), HipSvBackend(seed=1, precision=32)]
-    specs = W.planner_corpus(args.train + args.held_out, args.seed, n_min=args.n_min, n_max=args.n_max,
-                             d_max=args.d_max)
-    train, held = specs[:args.train], specs[args.train:]
-    t0 = time.time()
-    X, y = collect(backends, train, log=log, jit=args.jit)
-    log(f"collected {len(y)} launches in {time.time() - t0:.1f} s")
+    train, held = corpus_split(args.train, args.held_out, args.seed, args.n_min, args.n_max, args.d_max)
+    if args.command in ("collect", "all"):
+        a, b = (args.slice.split(":") + [""])[:2]
+        lo, hi = int(a or 0), int(b or len(train))
+        out = args.data[0] if args.data else None
+        deadline = time.time() + args.minutes * 60 if args.minutes > 0 else None
+        t0 = time.time()
+        # Backend alternation keeps the corpus index: circuit k is FP64 when k is even.
+        shifted = backends if lo % 2 == 0 else backends[::-1]
+        X, y = collect(shifted, train[lo:hi], log=log, seed=args.seed + lo, checkpoint=out, deadline=deadline,
+                       generated=args.generated, compiled=args.compiled)
+        log(f"collected {len(y)} launches in {time.time() - t0:.1f} s")
+        if args.command == "collect":
+            print(json.dumps({"launches": int(len(y)), "slice": [lo, hi], "data": out}))
+            return
+        Xs, ys = [X], [y]
+    else:
+        Xs, ys = [], []
+        for f in args.data:
+            d = np.load(f)
+            Xs.append(d["X"])
+            ys.append(d["y"])
+    X, y = np.concatenate(Xs), np.concatenate(ys)
     model = CostModel(backends[0], seed=args.seed)
     t0 = time.time()
     loss = float("nan")
     for e in range(0, args.epochs, 10):
         loss = model.train(X, y, epochs=min(10, args.epochs - e), batch=args.batch, lr=args.lr, seed=args.seed + e)
-        log(f"train: epoch {e + 10}/{args.epochs}, mse(ln us) {loss:.4f}")
+        log(f"train: epoch {min(e + 10, args.epochs)}/{args.epochs}, mse(ln us) {loss:.4f}")
     log(f"trained on the device in {time.time() - t0:.1f} s, mse(ln us) {loss:.4f}")
     twin = CostModel(backends[1], params=model.params())
-    rows = evaluate(backends, [model, twin], held, log=log, jit=args.jit)
+    rows = evaluate(backends, [model, twin], held, log=log)
     gain = float(np.exp(np.mean([math.log(r["rule_ms"] / r["learned_ms"]) for r in rows])))
-    model.meta = {"train_circuits": len(train), "launches": int(len(y)), "train_mse": loss,
+    model.meta = {"train_circuits": args.train, "launches": int(len(y)), "train_mse": loss,
                   "held_out": rows, "geomean_speedup_vs_rule": gain,
                   "corpus": {"seed": args.seed, "n": [args.n_min, args.n_max], "depth": [20, args.d_max],
-                             "precisions": [64, 32]},
-                  "codegen": "generated" if args.jit else "compiled"}
+                             "precisions": [64, 32],
+                             "candidates_per_circuit": "the rule options plus a seeded sample, as generated kernels "
+                                                       "and as the compiled op loop"},
+                  "codegen": "generated"}
     model.save(args.out)
     print(json.dumps({"launches": int(len(y)), "train_mse": loss, "geomean_speedup_vs_rule": gain}))
 

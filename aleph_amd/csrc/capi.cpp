@@ -21,6 +21,7 @@
 #include "gates.hpp"
 #include "jit.hpp"
 #include "kernels.hpp"
+#include "noise_host.hpp"
 
 using namespace aleph;
 
@@ -628,9 +629,16 @@ aleph_hip_status aleph_hip_apply_swap(aleph_hip_ctx* c, aleph_hip_state* st, uin
 
 namespace {
 // Candidate search of the learned planner: every combination of chunk
-// qubits, pass budget and 4x4 fusion is planned by the rule planner, the
-// cost model predicts each fused launch (ln microseconds), and the plan
-// with the smallest predicted total is kept.
+// qubits, pass budget, 4x4 fusion and non-temporal streaming is planned by
+// the rule planner, plus dense-block plans of width 3..6; when the caller
+// allows generated kernels (ALEPH_HIP_PLAN_JIT) each candidate is also
+// priced as the compiled op-loop kernel.  The cost model predicts every
+// fused launch (ln microseconds); standalone sweeps are priced at their
+// measured rates.  The cheapest candidate replaces the rule planner's own
+// plan only when it is predicted at least kLearnedMargin faster, so that
+// prediction noise cannot trade a good rule plan for a near-tie.
+constexpr double kLearnedMargin = 0.01;
+
 std::unique_ptr<Plan> plan_learned(aleph_hip_ctx* c, const std::vector<ProgInstr>& prog, uint32_t n,
                                    uint32_t precision, PlanOptions base) {
   const auto t0 = std::chrono::steady_clock::now();
@@ -639,12 +647,8 @@ std::unique_ptr<Plan> plan_learned(aleph_hip_ctx* c, const std::vector<ProgInstr
     bool f2q, nt;
     uint32_t kq;
     std::unique_ptr<Plan> p;
-    double us = 0.0;
+    double us[2] = {0.0, 0.0};  // predicted total: [0] compiled op loop, [1] generated kernels
   };
-  // Candidates: chunk qubits x pass budget x 4x4 pair fusion x non-temporal
-  // streaming, plus dense-block plans (k = 4, 5 on the matrix cores) at the
-  // rule planner's chunking.  The code variant (generated or compiled pass
-  // kernels) is the caller's (base.jit) and is a model feature.
   std::vector<Cand> cands;
   for (uint32_t m : {10u, 11u, 12u}) {
     if (m > n && m != 10u) continue;
@@ -658,18 +662,28 @@ std::unique_ptr<Plan> plan_learned(aleph_hip_ctx* c, const std::vector<ProgInstr
           po.fuse_2q = f2q;
           po.nontemporal = nt;
           po.kq_width = 0;
+          po.reg_bits = 0;
           cands.push_back(Cand{m, budget, f2q, nt, 0u, build_plan(prog, n, precision, po)});
         }
       }
     }
   }
-  for (uint32_t kq : {4u, 5u}) {
+  // The rule planner's own choice (12-qubit chunks, budget 24, 4x4 fusion).
+  size_t rule = 0;
+  for (size_t k = 0; k < cands.size(); k++)
+    if (std::min(cands[k].m, n) == std::min(12u, n) && cands[k].budget == 24u && cands[k].f2q && !cands[k].nt) {
+      rule = k;
+      break;
+    }
+  for (uint32_t kq = 3; kq <= (uint32_t)kMaxFusedK; kq++) {
     if (n < kq + 4) continue;
     PlanOptions po = base;
     po.planner = ALEPH_HIP_PLANNER_RULE;
     po.kq_width = kq;
+    po.reg_bits = 0;
     cands.push_back(Cand{std::min<uint32_t>(12u, n), 24u, true, false, kq, build_plan(prog, n, precision, po)});
   }
+  const int variants = base.jit ? 2 : 1;
   std::vector<float> rows;
   std::vector<uint32_t> owner;
   for (uint32_t k = 0; k < cands.size(); k++) {
@@ -677,34 +691,63 @@ std::unique_ptr<Plan> plan_learned(aleph_hip_ctx* c, const std::vector<ProgInstr
     const size_t before = rows.size() / kPlanFeatures;
     plan_features(*cands[k].p, rows, launch);
     owner.insert(owner.end(), rows.size() / kPlanFeatures - before, k);
-    cands[k].us = plan_standalone_us(*cands[k].p);
+    cands[k].us[0] = cands[k].us[1] = plan_standalone_us(*cands[k].p);
   }
   const uint32_t R = (uint32_t)owner.size();
   if (R) {
+    // Rows [0, R): the variant each candidate was built with (base.jit);
+    // rows [R, 2R): the same launches as the compiled op loop.
+    if (variants == 2) {
+      rows.resize(2 * rows.size());
+      std::copy(rows.begin(), rows.begin() + (size_t)R * kPlanFeatures, rows.begin() + (size_t)R * kPlanFeatures);
+      for (uint32_t r = 0; r < R; r++) rows[((size_t)R + r) * kPlanFeatures] = 0.f;
+    }
+    const uint32_t total = R * (uint32_t)variants;
     DevBuf dx(rows.size() * sizeof(float));
-    DevBuf dy(R * sizeof(float));
+    DevBuf dy(total * sizeof(float));
     ALEPH_HIP_CHECK(hipMemcpyAsync(dx.p, rows.data(), rows.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    launch_mlp_forward(c->stream, c->d_model, (const float*)dx.p, R, (float*)dy.p);
-    std::vector<float> pred(R);
-    ALEPH_HIP_CHECK(hipMemcpyAsync(pred.data(), dy.p, R * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    launch_mlp_forward(c->stream, c->d_model, (const float*)dx.p, total, (float*)dy.p);
+    std::vector<float> pred(total);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(pred.data(), dy.p, total * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
     const double scale = std::ldexp(1.0, (int)n - kCostRefQubits);
-    for (uint32_t r = 0; r < R; r++) cands[owner[r]].us += scale * std::exp((double)std::min(pred[r], 40.f));
+    for (uint32_t r = 0; r < R; r++) {
+      const double t = scale * std::exp((double)std::min(pred[r], 40.f));
+      cands[owner[r]].us[base.jit ? 1 : 0] += t;
+      if (variants == 2) cands[owner[r]].us[0] += scale * std::exp((double)std::min(pred[R + r], 40.f));
+    }
   }
-  size_t best = 0;
-  for (size_t k = 1; k < cands.size(); k++)
-    if (cands[k].us < cands[best].us) best = k;
+  const int own = base.jit ? 1 : 0;
+  size_t best = rule;
+  int best_v = own;
+  for (size_t k = 0; k < cands.size(); k++)
+    for (int v = 0; v < 2; v++) {
+      if (variants == 1 && v != own) continue;
+      if (cands[k].us[v] < cands[best].us[best_v]) {
+        best = k;
+        best_v = v;
+      }
+    }
+  const double rule_us = cands[rule].us[own];
+  if (cands[best].us[best_v] > (1.0 - kLearnedMargin) * rule_us) {
+    best = rule;
+    best_v = own;
+  }
   std::unique_ptr<Plan> p = std::move(cands[best].p);
   p->opts.planner = ALEPH_HIP_PLANNER_LEARNED;
+  p->opts.jit = best_v == 1;
   aleph_hip_plan_stats& st = p->stats;
   st.planner_used = ALEPH_HIP_PLANNER_LEARNED;
   st.chosen_local_qubits = std::min(cands[best].m, n);
   st.chosen_max_ops = cands[best].budget;
   st.chosen_flags = (cands[best].f2q ? 0u : (uint32_t)ALEPH_HIP_PLAN_NO_FUSE_2Q) |
-                    (cands[best].nt ? (uint32_t)ALEPH_HIP_PLAN_NONTEMPORAL : 0u);
+                    (cands[best].nt ? (uint32_t)ALEPH_HIP_PLAN_NONTEMPORAL : 0u) |
+                    (best_v == 1 ? (uint32_t)ALEPH_HIP_PLAN_JIT : 0u);
   st.chosen_kq_width = cands[best].kq;
-  st.n_candidates = (uint32_t)cands.size();
-  st.predicted_us = cands[best].us;
+  st.chosen_rule = best == rule ? 1u : 0u;
+  st.n_candidates = (uint32_t)(cands.size() * (size_t)variants);
+  st.predicted_us = cands[best].us[best_v];
+  st.rule_predicted_us = rule_us;
   st.plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return p;
 }
@@ -1084,6 +1127,124 @@ aleph_hip_status aleph_hip_linear_xeb(aleph_hip_ctx* c, aleph_hip_state* st, dou
     ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
     if (c->h_pinned[2] != 0.0) throw make_failure(ALEPH_HIP_ERR_INVALID_STATE, "non-finite amplitude", "invalid state");
     *out = std::ldexp(c->h_pinned[0], (int)st->n) - 1.0;
+  });
+}
+
+aleph_hip_status aleph_hip_dist_alloc_rank(aleph_hip_ctx* c, uint32_t m, uint32_t precision, uint32_t rank,
+                                           aleph_hip_state** out) {
+  const aleph_hip_status s = aleph_hip_state_alloc(c, m, precision, out);
+  if (s != ALEPH_HIP_OK) return s;
+  if (rank == 0) return aleph_hip_state_reset(c, *out);
+  return guarded(c, [&] {
+    ALEPH_HIP_CHECK(hipMemsetAsync((*out)->d, 0, (precision == ALEPH_HIP_F64 ? 16ull : 8ull) << m, c->stream));
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+aleph_hip_status aleph_hip_gate_matrix(const aleph_hip_gate* g, uint32_t num_qubits, double* out, uint32_t* dim) {
+  if (!g || !out || !dim) return ALEPH_HIP_ERR_INVALID_ARGUMENT;
+  try {
+    const DenseGate d = validate_gate(*g, num_qubits);
+    *dim = d.dim;
+    for (size_t e = 0; e < d.m.size(); e++) {
+      out[2 * e] = d.m[e].real();
+      out[2 * e + 1] = d.m[e].imag();
+    }
+    return ALEPH_HIP_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+aleph_hip_status aleph_hip_dist_exchange(aleph_hip_ctx* c, aleph_hip_state* const* ranks, uint32_t n_ranks,
+                                         const uint32_t* global_bits, uint32_t k) {
+  return guarded(c, [&] {
+    need(c != nullptr && ranks != nullptr && global_bits != nullptr, "null argument");
+    need(n_ranks >= 2 && (n_ranks & (n_ranks - 1)) == 0, "dist: rank count must be a power of two >= 2");
+    const uint32_t g = (uint32_t)__builtin_ctz(n_ranks);
+    for (uint32_t r = 0; r < n_ranks; r++) check_state(c, ranks[r]);
+    const uint32_t m = ranks[0]->n, prec = ranks[0]->precision;
+    for (uint32_t r = 1; r < n_ranks; r++)
+      need(ranks[r]->n == m && ranks[r]->precision == prec && ranks[r] != ranks[0], "dist: rank slices differ");
+    for (uint32_t a = 0; a < n_ranks; a++)
+      for (uint32_t b = a + 1; b < n_ranks; b++) need(ranks[a] != ranks[b], "dist: a rank slice appears twice");
+    need(k >= 1 && k <= m && k <= g && k <= kMaxExchangeBits, "dist: bad exchange bits");
+    ExchangeBits bits{};
+    uint32_t seen = 0;
+    for (uint32_t j = 0; j < k; j++) {
+      need(global_bits[j] >= m && global_bits[j] < m + g, "dist: bad exchange bits");
+      const uint32_t rb = global_bits[j] - m;
+      need(!((seen >> rb) & 1u), "dist: bad exchange bits");
+      seen |= 1u << rb;
+      bits.rank_bit[j] = rb;
+    }
+    std::vector<void*> ptrs(n_ranks);
+    for (uint32_t r = 0; r < n_ranks; r++) ptrs[r] = ranks[r]->d;
+    DevBuf dptr(n_ranks * sizeof(void*));
+    ALEPH_HIP_CHECK(hipMemcpyAsync(dptr.p, ptrs.data(), n_ranks * sizeof(void*), hipMemcpyHostToDevice, c->stream));
+    if (prec == ALEPH_HIP_F64) launch_exchange<double>(c->stream, (void* const*)dptr.p, n_ranks, m, k, bits);
+    else launch_exchange<float>(c->stream, (void* const*)dptr.p, n_ranks, m, k, bits);
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+aleph_hip_status aleph_hip_run_noisy(aleph_hip_ctx* c, uint32_t n, const uint32_t* words, size_t n_words,
+                                     const double* reals, size_t n_reals, const aleph_hip_channel* channels,
+                                     uint32_t n_channels, const double* readout, uint64_t readout_mask,
+                                     uint32_t shots, uint64_t seed, uint64_t shot_offset, size_t budget_bytes,
+                                     uint64_t* outcomes) {
+  return guarded(c, [&] {
+    need(c != nullptr, "null context");
+    need(shots == 0 || outcomes != nullptr, "null outcomes");
+    need(n_words == 0 || words != nullptr, "null words");
+    need(n_reals == 0 || reals != nullptr, "null reals");
+    if (n < 1 || n > kNoiseMaxQubits) {
+      Failure f = make_failure(ALEPH_HIP_ERR_TOO_MANY_QUBITS, "run_noisy", "noise trajectories need 1..24 qubits");
+      f.expected = kNoiseMaxQubits;
+      f.got = n;
+      throw f;
+    }
+    const std::vector<ProgInstr> prog = decode_program(words, n_words, reals, n_reals, n);
+    const TrajHost h = build_trajectories(prog, n, channels, n_channels, readout, readout_mask);
+    if (shots == 0) return;
+    uint32_t workgroups = 0;
+    size_t slice_bytes = 0;
+    trajectory_geometry(n, shots, budget_bytes ? budget_bytes : (size_t{4} << 30), &workgroups, &slice_bytes);
+    const size_t op_bytes = h.ops.size() * sizeof(TrajOp), real_bytes = h.reals.size() * sizeof(double);
+    const size_t mask_bytes = h.masks.size() * sizeof(uint64_t), index_bytes = h.index.size() * sizeof(uint32_t);
+    const size_t ro_bytes = std::max<size_t>(h.readout.size(), 1) * sizeof(double);
+    // One device block: program records, payloads, readout table, outcomes.
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_ops = 0, o_reals = al(op_bytes), o_masks = o_reals + al(real_bytes),
+                 o_index = o_masks + al(mask_bytes), o_ro = o_index + al(index_bytes), o_out = o_ro + al(ro_bytes),
+                 total = o_out + al((size_t)shots * sizeof(uint64_t));
+    DevBuf blk(total);
+    DevBuf slices(slice_bytes);
+    unsigned char* base = (unsigned char*)blk.p;
+    if (op_bytes)
+      ALEPH_HIP_CHECK(hipMemcpyAsync(base + o_ops, h.ops.data(), op_bytes, hipMemcpyHostToDevice, c->stream));
+    ALEPH_HIP_CHECK(hipMemcpyAsync(base + o_reals, h.reals.data(), real_bytes, hipMemcpyHostToDevice, c->stream));
+    ALEPH_HIP_CHECK(hipMemcpyAsync(base + o_masks, h.masks.data(), mask_bytes, hipMemcpyHostToDevice, c->stream));
+    ALEPH_HIP_CHECK(hipMemcpyAsync(base + o_index, h.index.data(), index_bytes, hipMemcpyHostToDevice, c->stream));
+    if (!h.readout.empty())
+      ALEPH_HIP_CHECK(hipMemcpyAsync(base + o_ro, h.readout.data(), h.readout.size() * sizeof(double),
+                                     hipMemcpyHostToDevice, c->stream));
+    TrajProgram tp;
+    tp.ops = (const TrajOp*)(base + o_ops);
+    tp.n_ops = (uint32_t)h.ops.size();
+    tp.n = n;
+    tp.reals = (const double*)(base + o_reals);
+    tp.masks = (const uint64_t*)(base + o_masks);
+    tp.index = (const uint32_t*)(base + o_index);
+    tp.readout = (const double*)(base + o_ro);
+    tp.readout_mask = h.readout_mask;
+    tp.seed = seed;
+    tp.shot0 = shot_offset;
+    tp.shots = shots;
+    uint64_t* d_out = (uint64_t*)(base + o_out);
+    launch_trajectories(c->stream, tp, workgroups, slices.p, d_out);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(outcomes, d_out, (size_t)shots * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
   });
 }
 

@@ -37,16 +37,17 @@ struct Failure {
 // Raises kernel `fn`'s dynamic LDS limit to the CU's 160 KiB, once per
 // (device, kernel).  Thread-safe: plans are built and launched from several
 // host threads, and a process may drive more than one device.
-inline void allow_full_lds(const void* fn) {
+inline void allow_full_lds(const void* fn, int bytes = 160 * 1024) {
   static std::mutex mu;
   static std::set<std::pair<int, const void*>> done;
   int dev = 0;
   ALEPH_HIP_CHECK(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lock(mu);
   if (!done.insert({dev, fn}).second) return;
-  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (e != hipSuccess) {
     done.erase({dev, fn});
+    (void)hipGetLastError();  // a failed call is also the thread's last error: do not leak it into later checks
     ALEPH_HIP_CHECK(e);
   }
 }

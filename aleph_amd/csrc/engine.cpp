@@ -1213,10 +1213,15 @@ void plan_features(const Plan& p, std::vector<float>& rows, std::vector<uint32_t
         if (op.type != OP_SIGN && op.type != OP_PHASE && op.cmask != 0) nctl += 1;
       }
     }
-    // The qubit count is not a feature: the model predicts a launch on
-    // 2^kCostRefQubits amplitudes and the planner scales by the state size,
-    // so predictions hold at qubit counts outside the training corpus.
-    // Slot 0 is the code variant (1: kernel generated for the pass).
+    // The model predicts a launch on 2^kCostRefQubits amplitudes and the
+    // planner scales by the state size.  Slot 14 still carries the state's
+    // size, log2(bytes) relative to 256 MiB (MI355X's last-level Infinity
+    // Cache): a state that stays cache-resident between passes streams
+    // faster, so a large pass budget that pays at n = 24 can be
+    // compute-bound at n = 30.  Slot 0 is the code variant (1: kernel
+    // generated for the pass).  The register width (reg_bits) is not a
+    // feature: every learned candidate uses the default of 4.
+    const float log2_bytes = (float)p.n + (p.precision == ALEPH_HIP_F64 ? 4.f : 3.f);
     const float f[kPlanFeatures] = {p.opts.jit ? 1.f : 0.f,
                                     p.precision == ALEPH_HIP_F64 ? 1.f : 0.f,
                                     (float)pd.m / 13.f,
@@ -1231,7 +1236,7 @@ void plan_features(const Plan& p, std::vector<float>& rows, std::vector<uint32_t
                                     (pd.features & kFeat2Q) ? 1.f : 0.f,
                                     (pd.features & kFeatPhase) ? 1.f : 0.f,
                                     (pd.features & kFeatNontemporal) ? 1.f : 0.f,
-                                    (float)pd.reg_bits - 4.f,
+                                    (log2_bytes - 28.f) / 6.f,
                                     nctl / 32.f};
     rows.insert(rows.end(), f, f + kPlanFeatures);
     launch.push_back((uint32_t)i);

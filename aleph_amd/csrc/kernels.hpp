@@ -40,6 +40,16 @@ void launch_phase_poly(hipStream_t s, void* state, uint32_t n, uint32_t n_terms,
 template <typename T>
 void launch_permute(hipStream_t s, const void* src, void* dst, uint32_t n, const uint32_t* perm);
 void launch_widen_f32(hipStream_t s, const void* src_f32, void* dst_f64, uint64_t len);
+
+// ---- rank-slice exchange of a distributed state on one device (k_exchange.hip) ----
+constexpr uint32_t kMaxExchangeBits = 16;
+struct ExchangeBits {
+  uint32_t rank_bit[kMaxExchangeBits];  // global_bits[j] - m
+};
+// d_ranks: device array of n_ranks slice pointers (2^m amplitudes each).
+template <typename T>
+void launch_exchange(hipStream_t s, void* const* d_ranks, uint32_t n_ranks, uint32_t m, uint32_t k,
+                     const ExchangeBits& bits);
 void launch_narrow_f64(hipStream_t s, const void* src_f64, void* dst_f32, uint64_t len);
 
 // ---- fused pass (k_fused.hip) ----

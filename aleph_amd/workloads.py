@@ -260,3 +260,19 @@ def parse_qasm(src: str) -> Circuit:
 
 def load_qasm(path) -> Circuit:
     return parse_qasm(Path(path).read_text())
+
+
+def noisy_brickwall(n: int, depth: int, seed: int = 0):
+    """The seeded brickwall with a device-style noise model: depolarizing
+    1e-3 after every U3 and 1e-2 after every CZ, amplitude damping 2e-3
+    after every U3 (a Kraus channel, state-dependent), 2 % / 3 % readout
+    error on every qubit.  Returns (circuit, NoiseModel)."""
+    from . import noise as N
+    c = brickwall(n, depth, seed=seed)
+    m = N.NoiseModel()
+    m.add_all_qubit_quantum_error(N.depolarizing_error(1e-3, 1), ["U3"])
+    m.add_all_qubit_quantum_error(N.depolarizing_error(1e-2, 2), ["Cz"])
+    m.add_all_qubit_quantum_error(N.amplitude_damping_error(2e-3), ["U3"])
+    for q in range(n):
+        m.add_readout_error(N.ReadoutError([[0.98, 0.02], [0.03, 0.97]]), q)
+    return c, m

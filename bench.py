@@ -60,12 +60,15 @@ def parse():
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 traffic measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched QAOA leg")
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--no-noise", action="store_true", help="skip the noise-trajectory leg")
+    ap.add_argument("--no-dist", action="store_true", help="skip the rank-slice exchange leg")
+    ap.add_argument("--noise-shots", type=int, default=100_000)
     ap.add_argument("--no-jit", action="store_true",
                     help="run fused passes with the compiled op-loop kernel instead of per-pass generated kernels")
     ap.add_argument("--traffic-child", action="store_true",
                     help="only reset, execute one plan and take the norm (the process rocprofv3 profiles)")
     ap.add_argument("--child-precision", type=int, default=64, choices=(32, 64))
-    ap.add_argument("--child-workload", default="brickwall", choices=("brickwall", "qft"))
+    ap.add_argument("--child-workload", default="brickwall", choices=("brickwall", "qft", "noise", "exchange"))
     return ap.parse_args()
 
 
@@ -194,6 +197,19 @@ def traffic_child(args) -> None:
     plan (FP64 brickwall by default)."""
     from aleph_amd import workloads as W
     from aleph_amd.backend import HipSvBackend
+    if args.child_workload == "noise":
+        from aleph_amd import noise as N
+        c, m = W.noisy_brickwall(NOISE_N, NOISE_DEPTH, seed=args.seed)
+        N.outcomes(c, m, args.noise_shots, 7, backend=HipSvBackend(seed=1))
+        return
+    if args.child_workload == "exchange":
+        from aleph_amd import dist as D
+        layout = D.DistLayout(args.n, 3)
+        dx = D.DistSvBackend(HipSvBackend(seed=1))
+        ranks = dx.alloc_ranks(layout)
+        for k in (1, 2, 3):
+            dx.exchange(ranks, layout, list(range(args.n - 1, args.n - 1 - k, -1)))
+        return
     b = HipSvBackend(seed=1, precision=args.child_precision)
     c = W.qft(args.n) if args.child_workload == "qft" else W.brickwall(args.n, args.depth, seed=args.seed)
     plan = b.compile(c, local_qubits=args.local_qubits, max_ops=args.max_ops, jit=not args.no_jit)
@@ -293,6 +309,67 @@ def batch_leg(args) -> dict:
             "config": "QAOA MaxCut, seeded random 3-regular graphs, n even in [12,24], p=4, FP32, <C> FP64"}
 
 
+NOISE_N, NOISE_DEPTH = 12, 10
+
+
+def noise_leg(args) -> dict:
+    """Noise trajectories: the seeded 12-qubit brickwall of depth 10 under
+    depolarizing, amplitude-damping and readout noise (workloads.noisy_brickwall),
+    one trajectory per shot, each in one workgroup with the state in LDS."""
+    from aleph_amd import noise as N
+    from aleph_amd import workloads as W
+    from aleph_amd.backend import HipSvBackend
+    c, m = W.noisy_brickwall(NOISE_N, NOISE_DEPTH, seed=args.seed)
+    hip = HipSvBackend(seed=1)
+    N.outcomes(c, m, 1024, 1, backend=hip)                      # warm-up
+    times = []
+    for rep in range(3):
+        t0 = time.perf_counter()
+        out = N.outcomes(c, m, args.noise_shots, 7 + rep, backend=hip)
+        times.append(time.perf_counter() - t0)
+    s = min(times)
+    _, att = N.resolve(c, m)
+    return {"workload": f"noisy brickwall n={NOISE_N} depth={NOISE_DEPTH}: depolarizing 1e-3 (U3) / 1e-2 (CZ), "
+                        f"amplitude damping 2e-3 (U3), readout 2-3 %", "shots": args.noise_shots, "s": s,
+            "trajectories_per_s": args.noise_shots / s, "gates": len(c.gates()), "channels_per_shot": len(att),
+            "distinct_outcomes": int(np.unique(out).size),
+            "note": "host-timed call: program upload, all trajectories, outcome copy back"}
+
+
+def exchange_leg(args) -> dict:
+    """Rank-slice exchange of a distributed n=30 FP64 state held as 8 ranks of
+    2^27 amplitudes on this device: k = 1, 2, 3 global bits swapped with the
+    top local bits in one in-place kernel.  Moved amplitudes are read once
+    and written once: 2 * (1 - 2^-k) * 2^30 * 16 bytes per exchange."""
+    from aleph_amd import dist as D
+    from aleph_amd import workloads as W
+    from aleph_amd.backend import HipSvBackend
+    n, g = 30, 3
+    layout = D.DistLayout(n, g)
+    hip = HipSvBackend(seed=1)
+    dx = D.DistSvBackend(hip)
+    ranks = dx.alloc_ranks(layout)
+    rows = []
+    for bits in ([n - 1], [n - 1, n - 2], [n - 1, n - 2, n - 3]):
+        k = len(bits)
+        dx.exchange(ranks, layout, bits)
+        best = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            dx.exchange(ranks, layout, bits)
+            best = min(best, time.perf_counter() - t0)
+        moved = 2.0 * (1.0 - 2.0 ** -k) * float(1 << n) * 16.0
+        rows.append({"k": k, "ms": best * 1e3, "gbs": moved / best / 1e9,
+                     "frac_of_hbm_peak": moved / best / 1e9 / HBM_PEAK_GBS})
+    for st in ranks:
+        st.free()
+    p = D.plan(W.brickwall(n, args.depth, seed=args.seed), layout)
+    return {"layout": "n=30 FP64, 8 ranks x 2^27 amplitudes on one device", "exchanges": rows,
+            "brickwall_plan": {"exchanges": p.stats.exchanges, "amps_moved_per_rank": p.stats.amps_moved_per_rank,
+                               "local_swaps": p.stats.local_swaps},
+            "note": "host-timed synchronous call (kernel + pointer upload); bytes = moved amplitudes read + written"}
+
+
 def main():
     args = parse()
     if args.traffic_child:
@@ -388,6 +465,13 @@ def main():
         log("bench: batched qaoa")
         out["qaoa_batch"] = batch_leg(args)
         log(f"bench: qaoa batch {out['qaoa_batch']['ms']:.1f} ms")
+    if not args.no_noise:
+        log("bench: noise trajectories")
+        out["noise"] = noise_leg(args)
+        log(f"bench: noise {out['noise']['trajectories_per_s']:.0f} trajectories/s")
+    if not args.no_dist:
+        log("bench: rank-slice exchange")
+        out["dist_exchange"] = exchange_leg(args)
     if not args.no_pmc:
         log("bench: rocprofv3 traffic passes")
         tr = measure_traffic(args, float(1 << n) * 16.0)

@@ -46,6 +46,14 @@ fn message(e: &ffi::aleph_hip_error) -> String {
 
 /// `aleph_hip_status` -> `BackendError`, field for field, in the header's
 /// enum order (crates/aleph-backend/src/lib.rs:22-81).
+/// The `BackendError` of a failed call (status != OK).
+pub(crate) fn status_error(ctx: *const ffi::aleph_hip_ctx, status: ffi::aleph_hip_status) -> BackendError {
+    match check(ctx, status) {
+        Err(e) => e,
+        Ok(()) => BackendError::DeviceFailure("status error without a failure".to_string()),
+    }
+}
+
 fn check(ctx: *const ffi::aleph_hip_ctx, status: ffi::aleph_hip_status) -> Result<(), BackendError> {
     if status == ffi::ALEPH_HIP_OK {
         return Ok(());
@@ -333,6 +341,10 @@ impl HipSvBackend {
     pub fn with_plan_options(mut self, plan: HipPlanOptions) -> Self {
         self.plan = plan;
         self
+    }
+
+    pub(crate) fn raw_ctx(&self) -> *mut ffi::aleph_hip_ctx {
+        self.ctx
     }
 
     fn opts(&self) -> ffi::aleph_hip_plan_opts {

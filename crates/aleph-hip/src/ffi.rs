@@ -80,6 +80,9 @@ pub const ALEPH_HIP_BATCH_PERSISTENT: u32 = 0;
 pub const ALEPH_HIP_BATCH_ROUNDS: u32 = 1;
 pub const ALEPH_HIP_BATCH_NONTEMPORAL: u32 = 1;
 
+pub const ALEPH_HIP_CHANNEL_PAULI: u32 = 0;
+pub const ALEPH_HIP_CHANNEL_KRAUS: u32 = 1;
+
 pub const ALEPH_HIP_MODEL_INPUTS: usize = 16;
 pub const ALEPH_HIP_MODEL_PARAMS: usize = 9473;
 
@@ -146,7 +149,8 @@ pub struct aleph_hip_plan_stats {
     pub n_candidates: u32,
     pub predicted_us: f64,
     pub chosen_kq_width: u32,
-    pub reserved: u32,
+    pub chosen_rule: u32,
+    pub rule_predicted_us: f64,
 }
 
 #[repr(C)]
@@ -204,6 +208,19 @@ pub struct aleph_hip_batch_view {
     pub round_begin: *const u32,
     pub threads: u32, pub lds_bytes: u32,
     pub total_amps: u64,
+}
+
+/// One noise channel attached after a program record (noise trajectories).
+#[repr(C)]
+pub struct aleph_hip_channel {
+    pub after: u32,
+    pub kind: u32,
+    pub n_qubits: u32,
+    pub n_branches: u32,
+    pub qubits: [u32; 2],
+    pub weights: *const f64,
+    pub paulis: *const u8,
+    pub kraus: *const f64,
 }
 
 extern "C" {
@@ -272,6 +289,20 @@ extern "C" {
     pub fn aleph_hip_batch_free(b: *mut aleph_hip_batch);
     pub fn aleph_hip_batch_build_host_qaoa(n_instances: u32, num_qubits: *const u32, edge_offsets: *const u64, edges: *const u32, p: u32, gammas: *const f64, betas: *const f64, precision: u32, opts: *const aleph_hip_batch_opts, out: *mut *mut aleph_hip_batch, err: *mut aleph_hip_error) -> aleph_hip_status;
     pub fn aleph_hip_batch_view_get(b: *const aleph_hip_batch, out: *mut aleph_hip_batch_view) -> aleph_hip_status;
+
+    pub fn aleph_hip_dist_alloc_rank(
+        ctx: *mut aleph_hip_ctx, m: u32, precision: u32, rank: u32, out: *mut *mut aleph_hip_state
+    ) -> aleph_hip_status;
+    pub fn aleph_hip_gate_matrix(g: *const aleph_hip_gate, num_qubits: u32, out: *mut f64, dim: *mut u32)
+        -> aleph_hip_status;
+    pub fn aleph_hip_dist_exchange(
+        ctx: *mut aleph_hip_ctx, ranks: *const *mut aleph_hip_state, n_ranks: u32, global_bits: *const u32, k: u32
+    ) -> aleph_hip_status;
+    pub fn aleph_hip_run_noisy(
+        ctx: *mut aleph_hip_ctx, num_qubits: u32, words: *const u32, n_words: usize, reals: *const f64,
+        n_reals: usize, channels: *const aleph_hip_channel, n_channels: u32, readout: *const f64,
+        readout_mask: u64, shots: u32, seed: u64, shot_offset: u64, budget_bytes: usize, outcomes: *mut u64
+    ) -> aleph_hip_status;
 }
 
 #[cfg(test)]
@@ -284,5 +315,6 @@ mod tests {
         assert_eq!(std::mem::size_of::<aleph_hip_gate>(), 64);
         assert_eq!(std::mem::size_of::<aleph_hip_plan_opts>(), 32);
         assert_eq!(std::mem::size_of::<aleph_hip_batch_opts>(), 32);
+        assert_eq!(std::mem::size_of::<aleph_hip_channel>(), 48);
     }
 }

@@ -25,6 +25,10 @@ pub mod ffi;
 #[cfg(all(target_os = "linux", feature = "hip"))]
 mod backend;
 
+/// `run_noisy` / `run_noisy_sparse` on the device (feature `noise`).
+#[cfg(all(target_os = "linux", feature = "noise"))]
+pub mod noise;
+
 #[cfg(all(target_os = "linux", feature = "hip"))]
 pub use backend::{
     device_available, encode_program, HipPlanOptions, HipState, HipSvBackend, HipSvBackendF32, HipSvReadout,

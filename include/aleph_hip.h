@@ -202,7 +202,8 @@ typedef struct aleph_hip_plan_stats {
   uint32_t chosen_local_qubits, chosen_max_ops, chosen_flags, n_candidates;
   double predicted_us;
   uint32_t chosen_kq_width;  /* LEARNED: dense-block width of the chosen candidate (0 = LDS passes) */
-  uint32_t reserved;
+  uint32_t chosen_rule;      /* LEARNED: 1 when the rule planner's own options were kept */
+  double rule_predicted_us;  /* LEARNED: predicted time of the rule planner's plan */
 } aleph_hip_plan_stats;
 
 aleph_hip_status aleph_hip_plan_create(aleph_hip_ctx* ctx, uint32_t num_qubits, uint32_t precision,
@@ -414,6 +415,59 @@ typedef struct aleph_hip_batch_view {
   uint64_t total_amps;
 } aleph_hip_batch_view;
 aleph_hip_status aleph_hip_batch_view_get(const aleph_hip_batch* b, aleph_hip_batch_view* out);
+
+/* ---- distributed state on one device: DistStep::Exchange through LocalExchange
+ * (crates/aleph-cuda/src/dist/exchange.rs:101-200; CPU form exchange_cpu, crates/aleph-sv/src/dist_ref.rs:78-104) ----
+ * ranks[r] is rank r's slice (2^m amplitudes, all of one precision); global physical bit
+ * global_bits[j] (m <= bit < m + log2(n_ranks)) trades places with local bit m-k+j: chunk c of
+ * rank r moves to rank r with bits global_bits[j]-m set to c's bits, at the chunk index made of r's
+ * old values of those bits.  In place, one kernel over every moved chunk pair. */
+/* DeviceSv::alloc_rank (crates/aleph-cuda/src/dist/mod.rs:33-35): an m-qubit rank slice,
+ * |0...0> for rank 0 and all-zero otherwise. */
+aleph_hip_status aleph_hip_dist_alloc_rank(aleph_hip_ctx* ctx, uint32_t m, uint32_t precision, uint32_t rank,
+                                           aleph_hip_state** out);
+/* Gate::matrix (crates/aleph-core/src/gate/kinds.rs:213-399) of a validated gate on a
+ * num_qubits-qubit register: out receives dim*dim interleaved complex, row-major, qubits[0] the
+ * MSB (at most 2*4^6 doubles).  Host only, no context. */
+aleph_hip_status aleph_hip_gate_matrix(const aleph_hip_gate* g, uint32_t num_qubits, double* out, uint32_t* dim);
+aleph_hip_status aleph_hip_dist_exchange(aleph_hip_ctx* ctx, aleph_hip_state* const* ranks, uint32_t n_ranks,
+                                         const uint32_t* global_bits, uint32_t k);
+
+/* ---- noise trajectories: run_noisy / run_noisy_sparse (crates/aleph-sv/src/noise/execution.rs:261-400) ----
+ * Monte-Carlo quantum-jump trajectories, one per shot, each from |0...0>.
+ * A channel fires after the program record `after` (a gate), channels of one
+ * record in array order (the all-qubit attachments first, then the
+ * qubit-specific ones: crates/aleph-sv/src/noise/model.rs:60-80).
+ * Pauli channels (PauliChannel, noise/error.rs:11-20) pick one Pauli string
+ * from `weights`, `paulis` holding n_qubits codes per branch (0 I, 1 X, 2 Y,
+ * 3 Z; code j acts on qubits[j]); Kraus channels (KrausChannel, one qubit)
+ * hold n_branches 2x2 complex matrices, interleaved row-major.  Readout
+ * errors (ReadoutError) give m[t][o] = P(read o | true t) as 4 doubles per
+ * qubit for the qubits set in `readout_mask`.
+ * Shot i of the call is absolute shot `shot_offset + i`: its outcome depends
+ * only on (seed, shot_offset + i), so a run split over several calls equals
+ * one call.  Outcomes are basis indices (bit q = qubit q).  States of up to
+ * 13 qubits run in LDS; 14..24 qubits run in per-workgroup device slices of
+ * at most `budget_bytes` (0: 4 GiB).  Errors: ALEPH_HIP_ERR_INVALID_ARGUMENT
+ * with kind "invalid-channel" (malformed channel or readout error, the
+ * reference's NoiseError::InvalidChannel) or "unsupported" (a dense gate on
+ * more than 3 qubits), ALEPH_HIP_ERR_TOO_MANY_QUBITS beyond 24 qubits. */
+enum { ALEPH_HIP_CHANNEL_PAULI = 0, ALEPH_HIP_CHANNEL_KRAUS = 1 };
+typedef struct aleph_hip_channel {
+  uint32_t after;
+  uint32_t kind;
+  uint32_t n_qubits;
+  uint32_t n_branches;
+  uint32_t qubits[2];
+  const double* weights;
+  const uint8_t* paulis;
+  const double* kraus;
+} aleph_hip_channel;
+aleph_hip_status aleph_hip_run_noisy(aleph_hip_ctx* ctx, uint32_t num_qubits, const uint32_t* words, size_t n_words,
+                                     const double* reals, size_t n_reals, const aleph_hip_channel* channels,
+                                     uint32_t n_channels, const double* readout, uint64_t readout_mask,
+                                     uint32_t shots, uint64_t seed, uint64_t shot_offset, size_t budget_bytes,
+                                     uint64_t* outcomes);
 
 #ifdef __cplusplus
 }

@@ -117,3 +117,73 @@ def qaoa_p1_cut_expectation(n: int, edges, gamma: float, beta: float) -> float:
         nb[v].add(u)
     return sum(qaoa_p1_edge_expectation(gamma, beta, len(nb[u]) - 1, len(nb[v]) - 1, len(nb[u] & nb[v]))
                for u, v in edges)
+
+
+# ---------------------------------------------------------------- noise
+
+def load_noise_fixture(name: str) -> dict:
+    """Exact Aer density-matrix distributions (reference oracle/noise/*.json)."""
+    return json.loads((GOLDEN / "noise" / f"{name}.json").read_text())
+
+
+def noise_fixture_case(name: str):
+    """(circuit, model) of one Aer noise fixture, as the reference's
+    crates/aleph-sv/tests/noise_oracle.rs:84-146 builds them: the damping
+    channels ride on an identity Unitary1q carrier after H."""
+    from aleph_amd import noise as N
+    from aleph_amd.circuit import Circuit, Gate
+    m = N.NoiseModel()
+    if name == "depol_h":
+        c = Circuit(1).h(0)
+        m.add_all_qubit_quantum_error(N.depolarizing_error(0.05, 1), ["H"])
+    elif name == "depol_cx":
+        c = Circuit(2).h(0).cx(0, 1)
+        m.add_quantum_error(N.depolarizing_error(0.1, 2), ["Cnot"], [0, 1])
+    elif name in ("amp_damp_h", "phase_damp_h"):
+        c = Circuit(1).h(0).gate(Gate.unitary(np.eye(2)), [0])
+        err = N.amplitude_damping_error(0.2) if name == "amp_damp_h" else N.phase_damping_error(0.3)
+        m.add_quantum_error(err, ["Unitary1q"], [0])
+    elif name == "readout_x":
+        c = Circuit(1).x(0)
+        m.add_readout_error(N.ReadoutError([[0.98, 0.02], [0.05, 0.95]]), 0)
+    elif name == "combined_ghz3":
+        c = Circuit(3).h(0).cx(0, 1).cx(1, 2)
+        m.add_all_qubit_quantum_error(N.depolarizing_error(0.02, 2), ["Cnot"])
+        for q in range(3):
+            m.add_readout_error(N.ReadoutError([[0.97, 0.03], [0.04, 0.96]]), q)
+    else:
+        raise KeyError(name)
+    return c, m
+
+
+NOISE_FIXTURES = ["depol_h", "depol_cx", "amp_damp_h", "phase_damp_h", "readout_x", "combined_ghz3"]
+
+
+def mixed_noise_case(n: int, depth: int, seed: int):
+    """A random circuit with every channel kind attached: depolarizing on
+    1q and 2q gates, amplitude and phase damping, bit/phase flips, a
+    weighted Pauli mixture, readout errors on some qubits."""
+    from aleph_amd import noise as N
+    from aleph_amd.circuit import Circuit, DiagonalPhase, Gate, PhaseTerm
+    rng = np.random.default_rng(seed)
+    c = Circuit(n)
+    for layer in range(depth):
+        for q in range(n):
+            c.u3(*rng.uniform(0, 2 * np.pi, 3), q)
+        for q in range(layer % 2, n - 1, 2):
+            c.cx(q, q + 1)
+        if layer == depth // 2:
+            c.ccx(0, 1, 2)
+            c.add(DiagonalPhase(n, [PhaseTerm((0b11,), 0.7), PhaseTerm((0b101, 0b110), -1.1)]))
+            c.gate(Gate.Rz(0.3), [n - 1], [0])
+    m = N.NoiseModel()
+    m.add_all_qubit_quantum_error(N.depolarizing_error(0.08, 2), ["Cnot"])
+    m.add_all_qubit_quantum_error(N.depolarizing_error(0.03, 1), ["U3"])
+    m.add_quantum_error(N.amplitude_damping_error(0.15), ["U3"], [1])
+    m.add_quantum_error(N.phase_damping_error(0.25), ["U3"], [2])
+    m.add_quantum_error(N.bit_flip_error(0.1), ["U3"], [0])
+    m.add_quantum_error(N.phase_flip_error(0.2), ["U3"], [n - 1])
+    m.add_quantum_error(N.pauli_error([("X", 1.0), ("Y", 2.0), ("I", 5.0)]), ["Rz"], [n - 1])
+    m.add_readout_error(N.ReadoutError([[0.9, 0.1], [0.2, 0.8]]), 0)
+    m.add_readout_error(N.ReadoutError([[0.97, 0.03], [0.06, 0.94]]), n - 1)
+    return c, m

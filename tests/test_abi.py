@@ -45,6 +45,7 @@ def test_struct_layouts():
     assert C.sizeof(L.BatchOpts) == 8 * 4
     assert C.sizeof(L.BatchStats) == 8 * 3 + 12 * 4
     assert C.sizeof(L.BatchTune) == 4 * 4 + 8
+    assert C.sizeof(L.Channel) == 6 * 4 + 3 * 8
 
 
 def test_no_device_paths_fail_cleanly():
