@@ -43,6 +43,8 @@ PLAN_NONTEMPORAL = 1
 PLAN_NO_FUSE_2Q = 2
 PLAN_NO_FUSE_DIAG = 4
 PLAN_JIT = 8
+PLAN_RENORMALIZE = 16
+PLAN_GENERIC_PHASE = 32
 INSTR_DIAGONAL_PHASE, INSTR_BARRIER = 64, 65
 
 
@@ -80,14 +82,14 @@ BATCH_UNREACHABLE_DEPS = 2
 
 class BatchOpts(C.Structure):
     _fields_ = [("mode", u32), ("local_qubits", u32), ("chunks_per_item", u32), ("groups_per_cu", u32),
-                ("max_ops_per_pass", u32), ("spin_limit", u32), ("flags", u32), ("reserved", u32)]
+                ("max_ops_per_pass", u32), ("spin_limit", u32), ("flags", u32), ("solo_qubits", u32)]
 
 
 class BatchStats(C.Structure):
     _fields_ = [("plan_ms", f64), ("sweep_bytes", f64), ("pool_bytes", u64), ("n_instances", u32),
                 ("n_passes", u32), ("n_items", u32), ("n_rounds", u32), ("grid", u32), ("threads", u32),
                 ("lds_bytes", u32), ("mode", u32), ("local_qubits", u32), ("chunks_per_item", u32),
-                ("groups_per_cu", u32), ("reserved", u32)]
+                ("groups_per_cu", u32), ("solo_instances", u32)]
 
 
 class BatchTune(C.Structure):
@@ -160,6 +162,8 @@ SIGNATURES = {
     "aleph_hip_expectation": (i32, [P, P, f64, PU32, PU8, u32, PF64]),
     "aleph_hip_expectation_sum": (i32, [P, P, u32, PF64, PU32, PU32, PU8, PF64]),
     "aleph_hip_linear_xeb": (i32, [P, P, PF64]),
+    "aleph_hip_state_normalize": (i32, [P, P, PF64]),
+    "aleph_hip_state_compare": (i32, [P, P, P, PF64]),
     "aleph_hip_probabilities": (i32, [P, P, PU32, u32, PF64]),
     "aleph_hip_batch_create": (i32, [P, u32, PU32, u32, PU32, PU64, PF64, PU64, C.POINTER(BatchOpts),
                                      C.POINTER(P)]),

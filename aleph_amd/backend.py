@@ -228,10 +228,12 @@ class HipSvBackend:
         self._check(L.load().aleph_hip_synchronize(self._ctx))
 
     # ---- Backend trait ----
-    def allocate(self, num_qubits: int) -> HipState:
+    def allocate(self, num_qubits: int, precision: int | None = None) -> HipState:
+        """A |0...0> state in the backend's precision (or `precision`, 64 or 32)."""
+        prec = self.precision if precision is None else precision
         h = C.c_void_p()
-        self._check(L.load().aleph_hip_state_alloc(self._ctx, num_qubits, self.precision, C.byref(h)))
-        return HipState(self, h, num_qubits, self.precision)
+        self._check(L.load().aleph_hip_state_alloc(self._ctx, num_qubits, prec, C.byref(h)))
+        return HipState(self, h, num_qubits, prec)
 
     def apply_gate(self, state: HipState, gate: GateInstance) -> None:
         g = L.Gate()
@@ -333,25 +335,43 @@ class HipSvBackend:
         self._check(L.load().aleph_hip_norm_sqr(self._ctx, state.handle, C.byref(out)))
         return float(out.value)
 
+    def normalize(self, state: HipState) -> float:
+        """psi /= sqrt(sum |psi|^2), the sum in FP64 on the device; returns the norm before."""
+        out = C.c_double()
+        self._check(L.load().aleph_hip_state_normalize(self._ctx, state.handle, C.byref(out)))
+        return float(out.value)
+
+    def compare(self, a: HipState, b: HipState) -> dict:
+        """<a|b>, fidelity |<a|b>|^2, sum |a-b|^2 and max |a-b| of two states of one size
+        (any precisions), accumulated in FP64 on the device."""
+        out = (C.c_double * 4)()
+        self._check(L.load().aleph_hip_state_compare(self._ctx, a.handle, b.handle, out))
+        ov = complex(out[0], out[1])
+        return {"overlap": ov, "fidelity": abs(ov) ** 2, "l2_sq": out[2], "max_abs": out[3]}
+
     # ---- plans ----
     def compile(self, circuit: Circuit, planner: int = L.PLANNER_RULE, local_qubits: int = 0, max_ops: int = 0,
                 coalesce_bits: int = 0, fuse_1q: bool = True, reg_bits: int = 0, nontemporal: bool = False,
-                fuse_2q: bool = True, fuse_diag: bool = True, kq_width: int = 0, jit: bool = False) -> HipPlan:
+                fuse_2q: bool = True, fuse_diag: bool = True, kq_width: int = 0, jit: bool = False,
+                renormalize: bool = False, generic_phase: bool = False, precision: int | None = None) -> HipPlan:
+        prec = self.precision if precision is None else precision
         words, reals = circuit.encode()
         nw, nr = int(words.size), int(reals.size)
         words = np.ascontiguousarray(words if nw else np.zeros(1, np.uint32))
         reals = np.ascontiguousarray(reals if nr else np.zeros(1, np.float64))
         opts = L.PlanOpts(planner, local_qubits, max_ops, coalesce_bits, int(fuse_1q), reg_bits,
                           (L.PLAN_NONTEMPORAL if nontemporal else 0) | (0 if fuse_2q else L.PLAN_NO_FUSE_2Q)
-                          | (0 if fuse_diag else L.PLAN_NO_FUSE_DIAG) | (L.PLAN_JIT if jit else 0), kq_width)
+                          | (0 if fuse_diag else L.PLAN_NO_FUSE_DIAG) | (L.PLAN_JIT if jit else 0)
+                          | (L.PLAN_RENORMALIZE if renormalize else 0)
+                          | (L.PLAN_GENERIC_PHASE if generic_phase else 0), kq_width)
         h = C.c_void_p()
-        self._check(L.load().aleph_hip_plan_create(self._ctx, circuit.num_qubits, self.precision,
+        self._check(L.load().aleph_hip_plan_create(self._ctx, circuit.num_qubits, prec,
                                                    _ptr(words, C.c_uint32), nw, _ptr(reals, C.c_double), nr,
                                                    C.byref(opts), C.byref(h)))
-        return HipPlan(self, h, circuit.num_qubits, self.precision)
+        return HipPlan(self, h, circuit.num_qubits, prec)
 
     def run_program(self, state: HipState, circuit: Circuit, **plan_kw) -> dict:
-        plan = self.compile(circuit, **plan_kw)
+        plan = self.compile(circuit, **{"precision": state.precision, **plan_kw})
         try:
             plan.execute(state)
             self.synchronize()

@@ -19,9 +19,10 @@ from . import _lib as L
 from .backend import HipSvBackend, _ptr, _raise
 
 
-def _opts(mode, local_qubits, chunks_per_item, groups_per_cu, max_ops, spin_limit, nontemporal, flags=0):
+def _opts(mode, local_qubits, chunks_per_item, groups_per_cu, max_ops, spin_limit, nontemporal, flags=0,
+          solo_qubits=0):
     return L.BatchOpts(mode, local_qubits, chunks_per_item, groups_per_cu, max_ops, spin_limit,
-                       (L.BATCH_NONTEMPORAL if nontemporal else 0) | flags, 0)
+                       (L.BATCH_NONTEMPORAL if nontemporal else 0) | flags, solo_qubits)
 
 
 def _edge_arrays(graphs):
@@ -48,15 +49,17 @@ class HipBatch:
     @classmethod
     def qaoa(cls, backend: HipSvBackend, graphs, gammas, betas, precision: int = L.F32, mode: int = L.BATCH_PERSISTENT,
              local_qubits: int = 0, chunks_per_item: int = 0, groups_per_cu: int = 0, max_ops: int = 0,
-             spin_limit: int = 0, nontemporal: bool = False, flags: int = 0) -> "HipBatch":
-        """graphs: list of (n, edges); gammas/betas: (instances, p) arrays."""
+             spin_limit: int = 0, nontemporal: bool = False, flags: int = 0, solo_qubits: int = 0) -> "HipBatch":
+        """graphs: list of (n, edges); gammas/betas: (instances, p) arrays.  Instances of at least
+        `solo_qubits` qubits (0: 21, above 40: none) run as their own plans with generated kernels."""
         nq = np.ascontiguousarray(np.array([g[0] for g in graphs], dtype=np.uint32))
         offs, flat = _edge_arrays(graphs)
         ga = np.ascontiguousarray(np.asarray(gammas, dtype=np.float64).reshape(len(graphs), -1))
         be = np.ascontiguousarray(np.asarray(betas, dtype=np.float64).reshape(len(graphs), -1))
         if ga.shape != be.shape:
             raise ValueError("gammas and betas must have the same shape")
-        o = _opts(mode, local_qubits, chunks_per_item, groups_per_cu, max_ops, spin_limit, nontemporal, flags)
+        o = _opts(mode, local_qubits, chunks_per_item, groups_per_cu, max_ops, spin_limit, nontemporal, flags,
+                  solo_qubits)
         h = C.c_void_p()
         backend._check(L.load().aleph_hip_batch_create_qaoa(
             backend._ctx, len(graphs), _ptr(nq, C.c_uint32), _ptr(offs, C.c_uint64), _ptr(flat, C.c_uint32),
@@ -66,7 +69,7 @@ class HipBatch:
     @classmethod
     def circuits(cls, backend: HipSvBackend, circuits, precision: int = L.F32, mode: int = L.BATCH_PERSISTENT,
                  local_qubits: int = 0, chunks_per_item: int = 0, groups_per_cu: int = 0, max_ops: int = 0,
-                 spin_limit: int = 0, nontemporal: bool = False) -> "HipBatch":
+                 spin_limit: int = 0, nontemporal: bool = False, solo_qubits: int = 0) -> "HipBatch":
         nq = np.ascontiguousarray(np.array([c.num_qubits for c in circuits], dtype=np.uint32))
         words, reals = [], []
         woff = np.zeros(len(circuits) + 1, dtype=np.uint64)
@@ -79,7 +82,8 @@ class HipBatch:
             roff[i + 1] = roff[i] + r.size
         W = np.ascontiguousarray(np.concatenate(words + [np.zeros(1, np.uint32)]).astype(np.uint32))
         R = np.ascontiguousarray(np.concatenate(reals + [np.zeros(1)]).astype(np.float64))
-        o = _opts(mode, local_qubits, chunks_per_item, groups_per_cu, max_ops, spin_limit, nontemporal)
+        o = _opts(mode, local_qubits, chunks_per_item, groups_per_cu, max_ops, spin_limit, nontemporal,
+                  solo_qubits=solo_qubits)
         h = C.c_void_p()
         backend._check(L.load().aleph_hip_batch_create(
             backend._ctx, len(circuits), _ptr(nq, C.c_uint32), precision, _ptr(W, C.c_uint32),

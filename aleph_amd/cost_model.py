@@ -16,10 +16,13 @@ smallest predicted time when it beats the rule planner's plan by at least
 
     python -m aleph_amd.cost_model collect --slice 0:75 --data a.npz
     python -m aleph_amd.cost_model collect --slice 75:150 --data b.npz
-    python -m aleph_amd.cost_model train --data a.npz b.npz --out cost_model.json
+    python -m aleph_amd.cost_model train --data a.npz b.npz --out cost_model.json --minutes 10
+    python -m aleph_amd.cost_model evaluate --model cost_model.json --slice 20:40 --out rows.json
+    python -m aleph_amd.cost_model merge --model cost_model.json --data rows.json --out cost_model.json
 
 times a seeded sample of candidates of each training circuit on the
-device, trains, evaluates on the held-out circuits and writes the model.
+device, trains on the device, times the rule and the learned plan of each
+held-out circuit and writes the model with those measurements.
 """
 
 from __future__ import annotations
@@ -245,17 +248,25 @@ def corpus_split(train: int, held_out: int, seed: int, n_min: int, n_max: int, d
     return specs[:train], specs[train:]
 
 
+def _rows_gain(rows) -> float:
+    return float(np.exp(np.mean([math.log(r["rule_ms"] / r["learned_ms"]) for r in rows]))) if rows else float("nan")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description="train the learned planner's cost model on this device")
-    ap.add_argument("command", choices=["collect", "train", "all"],
+    ap.add_argument("command", choices=["collect", "train", "evaluate", "merge"],
                     help="collect: time candidate plans of a slice of the training corpus into --data; "
-                         "train: train on --data files, evaluate on the held-out corpus, write --out; "
-                         "all: both in one process")
+                         "train: train on --data files, write --out, evaluate held-out circuits until --minutes; "
+                         "evaluate: time held-out circuits [--slice] with the model --model into --out (JSON rows); "
+                         "merge: add evaluated rows files (--data) to the model --model (no device needed)")
     ap.add_argument("--out", default=str(DEFAULT_PATH))
-    ap.add_argument("--data", nargs="*", default=[], help="collected .npz files (collect writes the first)")
+    ap.add_argument("--model", default=str(DEFAULT_PATH))
+    ap.add_argument("--data", nargs="*", default=[], help="collected .npz files (collect writes the first), "
+                                                         "or evaluated rows files for merge")
     ap.add_argument("--train", type=int, default=150, help="training circuits (alternating FP64 / FP32)")
     ap.add_argument("--held-out", type=int, default=40, help="evaluation circuits")
-    ap.add_argument("--slice", default="0:", help="collect only training circuits [a:b] of the corpus")
+    ap.add_argument("--slice", default="0:", help="the circuits [a:b] of the training (collect) or held-out "
+                                                  "(evaluate) corpus")
     ap.add_argument("--n-min", type=int, default=24)
     ap.add_argument("--n-max", type=int, default=30)
     ap.add_argument("--d-max", type=int, default=100)
@@ -263,34 +274,67 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--minutes", type=float, default=0.0, help="stop collecting after this long (0: no limit)")
+    ap.add_argument("--minutes", type=float, default=0.0, help="stop collecting / evaluating after this long")
     ap.add_argument("--generated", type=int, default=3, help="candidates timed per circuit as generated kernels")
     ap.add_argument("--compiled", type=int, default=3, help="candidates timed per circuit as the compiled op loop")
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
-    backends = [HipSvBackend(seed=1, precision=64), HipSvBackend(seed=1, precision=32)]
     train, held = corpus_split(args.train, args.held_out, args.seed, args.n_min, args.n_max, args.d_max)
-    if args.command in ("collect", "all"):
-        a, b = (args.slice.split(":") + [""])[:2]
+    a, b = (args.slice.split(":") + [""])[:2]
+    deadline = time.time() + args.minutes * 60 if args.minutes > 0 else None
+
+    if args.command == "merge":
+        d = json.loads(Path(args.model).read_text())
+        rows = list(d["meta"].get("held_out", []))
+        for f in args.data:
+            rows += json.loads(Path(f).read_text())
+        seen, uniq = set(), []
+        for r in rows:
+            key = json.dumps(r["spec"])
+            if key not in seen:
+                seen.add(key)
+                uniq.append(r)
+        d["meta"]["held_out"] = uniq
+        d["meta"]["held_out_evaluated"] = len(uniq)
+        d["meta"]["geomean_speedup_vs_rule"] = _rows_gain(uniq)
+        Path(args.out).write_text(json.dumps(d))
+        print(json.dumps({"held_out_evaluated": len(uniq), "geomean_speedup_vs_rule": _rows_gain(uniq)}))
+        return
+
+    backends = [HipSvBackend(seed=1, precision=64), HipSvBackend(seed=1, precision=32)]
+    if args.command == "collect":
         lo, hi = int(a or 0), int(b or len(train))
         out = args.data[0] if args.data else None
-        deadline = time.time() + args.minutes * 60 if args.minutes > 0 else None
         t0 = time.time()
         # Backend alternation keeps the corpus index: circuit k is FP64 when k is even.
         shifted = backends if lo % 2 == 0 else backends[::-1]
         X, y = collect(shifted, train[lo:hi], log=log, seed=args.seed + lo, checkpoint=out, deadline=deadline,
                        generated=args.generated, compiled=args.compiled)
         log(f"collected {len(y)} launches in {time.time() - t0:.1f} s")
-        if args.command == "collect":
-            print(json.dumps({"launches": int(len(y)), "slice": [lo, hi], "data": out}))
-            return
-        Xs, ys = [X], [y]
-    else:
-        Xs, ys = [], []
-        for f in args.data:
-            d = np.load(f)
-            Xs.append(d["X"])
-            ys.append(d["y"])
+        print(json.dumps({"launches": int(len(y)), "slice": [lo, hi], "data": out}))
+        return
+
+    if args.command == "evaluate":
+        lo, hi = int(a or 0), int(b or len(held))
+        model = CostModel.load(backends[0], args.model)
+        twin = CostModel(backends[1], params=model.params())
+        shifted = backends if lo % 2 == 0 else backends[::-1]
+        rows = []
+        for k in range(lo, hi):
+            if deadline is not None and time.time() > deadline:
+                log(f"evaluate: deadline reached after {len(rows)} circuits")
+                break
+            rows += evaluate(shifted[(k - lo) % 2:] + shifted[:(k - lo) % 2], [model, twin], [held[k]], log=log)
+            Path(args.out).write_text(json.dumps(rows))
+        print(json.dumps({"evaluated": len(rows), "geomean_speedup_vs_rule": _rows_gain(rows)}))
+        return
+
+    Xs, ys, circuits = [], [], 0
+    for f in args.data:
+        d = np.load(f)
+        Xs.append(d["X"])
+        ys.append(d["y"])
+        circuits += int(d["circuits"]) if "circuits" in d else 0
     X, y = np.concatenate(Xs), np.concatenate(ys)
     model = CostModel(backends[0], seed=args.seed)
     t0 = time.time()
@@ -299,18 +343,26 @@ def main(argv=None):
         loss = model.train(X, y, epochs=min(10, args.epochs - e), batch=args.batch, lr=args.lr, seed=args.seed + e)
         log(f"train: epoch {min(e + 10, args.epochs)}/{args.epochs}, mse(ln us) {loss:.4f}")
     log(f"trained on the device in {time.time() - t0:.1f} s, mse(ln us) {loss:.4f}")
-    twin = CostModel(backends[1], params=model.params())
-    rows = evaluate(backends, [model, twin], held, log=log)
-    gain = float(np.exp(np.mean([math.log(r["rule_ms"] / r["learned_ms"]) for r in rows])))
-    model.meta = {"train_circuits": args.train, "launches": int(len(y)), "train_mse": loss,
-                  "held_out": rows, "geomean_speedup_vs_rule": gain,
+    model.meta = {"train_circuits": circuits, "launches": int(len(y)), "train_mse": loss,
+                  "held_out": [], "held_out_evaluated": 0, "geomean_speedup_vs_rule": float("nan"),
                   "corpus": {"seed": args.seed, "n": [args.n_min, args.n_max], "depth": [20, args.d_max],
-                             "precisions": [64, 32],
+                             "precisions": [64, 32], "held_out_circuits": args.held_out,
                              "candidates_per_circuit": "the rule options plus a seeded sample, as generated kernels "
                                                        "and as the compiled op loop"},
                   "codegen": "generated"}
     model.save(args.out)
-    print(json.dumps({"launches": int(len(y)), "train_mse": loss, "geomean_speedup_vs_rule": gain}))
+    twin = CostModel(backends[1], params=model.params())
+    rows = []
+    for k in range(len(held)):
+        if deadline is not None and time.time() > deadline:
+            log(f"evaluate: deadline reached after {len(rows)} circuits")
+            break
+        rows += evaluate(backends[k % 2:] + backends[:k % 2], [model, twin], [held[k]], log=log)
+        model.meta.update({"held_out": rows, "held_out_evaluated": len(rows),
+                           "geomean_speedup_vs_rule": _rows_gain(rows)})
+        model.save(args.out)
+    print(json.dumps({"launches": int(len(y)), "train_mse": loss, "held_out_evaluated": len(rows),
+                      "geomean_speedup_vs_rule": _rows_gain(rows)}))
 
 
 if __name__ == "__main__":

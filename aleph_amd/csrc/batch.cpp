@@ -11,6 +11,7 @@
 #include <thread>
 
 #include "gates.hpp"
+#include "jit.hpp"
 
 namespace aleph {
 
@@ -113,6 +114,7 @@ void Batch::plan() {
   const auto t0 = std::chrono::steady_clock::now();
   const uint32_t N = size();
   const uint32_t m_cfg = cfg.local_qubits ? cfg.local_qubits : 12u;
+  const uint32_t solo_q = cfg.solo_qubits ? cfg.solo_qubits : kDefaultSoloQubits;
   if (m_cfg < kBatchRegBits + 1 || m_cfg > (uint32_t)kMaxLocalBits) {
     throw make_failure(ALEPH_HIP_ERR_INVALID_ARGUMENT, "batch", "local_qubits must be in [5, 13]");
   }
@@ -131,6 +133,12 @@ void Batch::plan() {
         po.max_ops = cfg.max_ops;
         po.reg_bits = kBatchRegBits;
         po.nontemporal = cfg.nontemporal;
+        const bool own = nq[i] >= solo_q;
+        if (own) {
+          po.local_qubits = 0;        // the single-state planner's chunking
+          po.jit = true;
+          po.generic_phase = true;    // instances of one size share their pass kernels
+        }
         std::unique_ptr<Plan> p;
         if (qaoa.empty()) {
           p = build_plan(progs[i], nq[i], precision, po);
@@ -141,7 +149,7 @@ void Batch::plan() {
                          nq[i], precision, po);
         }
         for (const Step& s : p->steps)
-          if (s.kind != Step::FUSED)
+          if (!own && s.kind != Step::FUSED)
             throw make_failure(ALEPH_HIP_ERR_UNSUPPORTED_INSTRUCTION, "batch",
                                "batched circuits must plan into fused passes only (no wide dense blocks or "
                                "wide parity terms)");
@@ -169,14 +177,23 @@ void Batch::plan() {
   offs.assign(N, 0);
   inst_passes.assign(N, 0);
   std::vector<std::vector<uint32_t>> pass_list(N);
+  solo.clear();
+  solo_inst.clear();
   uint64_t amp = 0;
   uint32_t m_max = kBatchRegBits;
   const double bpa = precision == ALEPH_HIP_F64 ? 16.0 : 8.0;
   sweep_bytes = 0.0;
   for (uint32_t i = 0; i < N; i++) {
-    const Plan& p = *plans[i];
     offs[i] = amp;
     amp += 1ull << nq[i];
+    if (nq[i] >= solo_q) {
+      inst_passes[i] = (uint32_t)plans[i]->steps.size();
+      sweep_bytes += (double)inst_passes[i] * 2.0 * std::ldexp(bpa, (int)nq[i]);
+      solo_inst.push_back(i);
+      solo.push_back(std::move(plans[i]));
+      continue;
+    }
+    const Plan& p = *plans[i];
     const uint32_t sb = (uint32_t)stages.size(), ob = (uint32_t)ops.size(), tb = (uint32_t)terms.size();
     for (const StageDev& sd : p.stages) {
       StageDev c = sd;
@@ -202,6 +219,8 @@ void Batch::plan() {
   }
   total_amps = amp;
   total_passes = (uint32_t)passes.size();
+  for (const auto& sp : solo) total_passes += (uint32_t)sp->steps.size();
+  if (!solo.empty()) solo_generation++;
 
   // Items, round-major; within a round the largest instances first.
   std::vector<uint32_t> order(N);
@@ -209,14 +228,15 @@ void Batch::plan() {
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return nq[a] > nq[b]; });
   const uint32_t cpi = cfg.chunks_per_item ? cfg.chunks_per_item : kDefaultChunksPerItem;
   uint32_t rounds = 0;
-  for (uint32_t i = 0; i < N; i++) rounds = std::max(rounds, inst_passes[i]);
+  for (uint32_t i = 0; i < N; i++)
+    if (nq[i] < solo_q) rounds = std::max(rounds, inst_passes[i]);
   items.clear();
   round_begin.clear();
   std::vector<uint32_t> dep(N, 0);
   for (uint32_t r = 0; r < rounds; r++) {
     round_begin.push_back((uint32_t)items.size());
     for (uint32_t i : order) {
-      if (r >= inst_passes[i]) continue;
+      if (nq[i] >= solo_q || r >= inst_passes[i]) continue;
       const uint32_t g = pass_list[i][r];
       const uint32_t chunks = 1u << (nq[i] - passes[g].m);
       for (uint32_t c = 0; c < chunks; c += cpi) {
@@ -274,6 +294,10 @@ void Batch::upload(hipStream_t s, int device) {
   }
   if (N) ALEPH_HIP_CHECK(hipMemcpyAsync(d_offs, offs.data(), N * sizeof(uint64_t), hipMemcpyHostToDevice, s));
   if (realloc) drop_graph();
+  for (auto& sp : solo) {
+    if (!sp->uploaded) sp->upload(s);
+    if (sp->jit.empty()) jit_compile_plan(*sp);
+  }
   int cus = 0;
   ALEPH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
   const int occ = precision == ALEPH_HIP_F64 ? batch_persistent_occupancy<double>(launch)
@@ -300,6 +324,8 @@ void record_batch(Batch& b, hipStream_t s) {
       launch_batch_round<T>(s, b.launch, b.pool, b.d_items + b.round_begin[r], b.round_begin[r + 1] - b.round_begin[r],
                             b.d_passes, b.d_stages, b.d_ops, b.d_terms);
   }
+  for (size_t k = 0; k < b.solo.size(); k++)
+    b.solo[k]->execute(s, static_cast<T*>(b.pool) + 2 * b.offs[b.solo_inst[k]]);
 }
 }  // namespace
 
@@ -309,6 +335,7 @@ Batch::GraphKey Batch::current_key() const {
   k.grid = grid;
   k.threads = launch.threads;
   k.n_items = (uint32_t)items.size();
+  k.solo_generation = solo_generation;
   k.lds_bytes = launch.lds_bytes;
   k.nontemporal = launch.nontemporal;
   k.round_begin = round_begin;

@@ -209,6 +209,14 @@ void reduce(aleph_hip_ctx* c, aleph_hip_state* st, int32_t q, double out[3]) {
   for (int i = 0; i < 3; i++) out[i] = c->h_pinned[i];
 }
 
+// FP64 norm of the state, then psi /= sqrt(norm), both on the context stream.
+void normalize_async(aleph_hip_ctx* c, aleph_hip_state* st) {
+  double* partials = c->d_scratch;
+  double* res = c->d_scratch + kResOffset;
+  if (st->precision == ALEPH_HIP_F64) launch_normalize<double>(c->stream, st->d, st->n, partials, res);
+  else launch_normalize<float>(c->stream, st->d, st->n, partials, res);
+}
+
 // validate_state (crates/aleph-sv/src/measure.rs:23-39).
 void validate(aleph_hip_ctx* c, aleph_hip_state* st, double* p0, double* p1, int32_t q) {
   double r[3];
@@ -772,6 +780,8 @@ std::unique_ptr<aleph_hip_plan> make_plan(aleph_hip_ctx* c, uint32_t n, uint32_t
       po.fuse_2q = (opts->flags & ALEPH_HIP_PLAN_NO_FUSE_2Q) == 0;
       po.fuse_diag = (opts->flags & ALEPH_HIP_PLAN_NO_FUSE_DIAG) == 0;
       po.kq_width = opts->kq_width;
+      po.renormalize = (opts->flags & ALEPH_HIP_PLAN_RENORMALIZE) != 0;
+      po.generic_phase = (opts->flags & ALEPH_HIP_PLAN_GENERIC_PHASE) != 0;
     }
     need(po.kq_width == 0 || (po.kq_width >= 3 && po.kq_width <= kMaxFusedK), "kq_width must be 0 or in [3, 6]");
     need(po.reg_bits == 0 || po.reg_bits == 4 || po.reg_bits == 5, "reg_bits must be 4 or 5");
@@ -854,6 +864,7 @@ aleph_hip_status aleph_hip_plan_execute(aleph_hip_ctx* c, aleph_hip_plan* plan, 
     need(plan->plan->uploaded, "plan was built on the host only");
     need(plan->plan->n == st->n && plan->plan->precision == st->precision, "plan built for another state shape");
     plan->plan->execute(c->stream, st->d);
+    if (plan->plan->opts.renormalize) normalize_async(c, st);
   });
 }
 
@@ -877,6 +888,7 @@ aleph_hip_status aleph_hip_plan_execute_timed(aleph_hip_ctx* c, aleph_hip_plan* 
     } guard{ev};
     for (auto& e : ev) ALEPH_HIP_CHECK(hipEventCreate(&e));
     p.execute_with_events(c->stream, st->d, ev.data());
+    if (p.opts.renormalize) normalize_async(c, st);
     ALEPH_HIP_CHECK(hipEventSynchronize(ev[ns]));
     for (size_t i = 0; i < ns; i++) {
       float t = 0.f;
@@ -916,6 +928,49 @@ aleph_hip_status aleph_hip_norm_sqr(aleph_hip_ctx* c, aleph_hip_state* st, doubl
     double r[3];
     reduce(c, st, -1, r);
     *out = r[2] != 0.0 ? NAN : r[0] + r[1];
+  });
+}
+
+aleph_hip_status aleph_hip_state_normalize(aleph_hip_ctx* c, aleph_hip_state* st, double* norm_before) {
+  return guarded(c, [&] {
+    check_state(c, st);
+    normalize_async(c, st);
+    if (norm_before) {
+      ALEPH_HIP_CHECK(hipMemcpyAsync(c->h_pinned, c->d_scratch + kResOffset, 3 * sizeof(double),
+                                     hipMemcpyDeviceToHost, c->stream));
+      ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+      *norm_before = c->h_pinned[2] != 0.0 ? NAN : c->h_pinned[0] + c->h_pinned[1];
+    }
+  });
+}
+
+aleph_hip_status aleph_hip_state_compare(aleph_hip_ctx* c, aleph_hip_state* a, aleph_hip_state* b, double* out) {
+  return guarded(c, [&] {
+    check_state(c, a);
+    check_state(c, b);
+    need(out != nullptr, "null output");
+    need(a->n == b->n, "states differ in size");
+    double* partials = c->d_scratch;
+    uint32_t blocks = 0;
+    const bool a64 = a->precision == ALEPH_HIP_F64, b64 = b->precision == ALEPH_HIP_F64;
+    if (a64 && b64) blocks = launch_compare<double, double>(c->stream, a->d, b->d, a->n, partials);
+    else if (a64) blocks = launch_compare<double, float>(c->stream, a->d, b->d, a->n, partials);
+    else if (b64) blocks = launch_compare<float, double>(c->stream, a->d, b->d, a->n, partials);
+    else blocks = launch_compare<float, float>(c->stream, a->d, b->d, a->n, partials);
+    std::vector<double> h(4 * (size_t)blocks);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(h.data(), partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
+    double re = 0, im = 0, d2 = 0, mx = 0;
+    for (uint32_t k = 0; k < blocks; k++) {
+      re += h[4 * k];
+      im += h[4 * k + 1];
+      d2 += h[4 * k + 2];
+      mx = std::max(mx, h[4 * k + 3]);
+    }
+    out[0] = re;
+    out[1] = im;
+    out[2] = d2;
+    out[3] = std::sqrt(mx);
   });
 }
 
@@ -1287,6 +1342,7 @@ BatchConfig batch_config(const aleph_hip_batch_opts* o) {
   c.spin_limit = o->spin_limit;
   c.nontemporal = (o->flags & ALEPH_HIP_BATCH_NONTEMPORAL) != 0;
   c.unreachable_deps = (o->flags & ALEPH_HIP_BATCH_UNREACHABLE_DEPS) != 0;
+  c.solo_qubits = o->solo_qubits;
   return c;
 }
 
@@ -1491,6 +1547,7 @@ aleph_hip_status aleph_hip_batch_stats_get(const aleph_hip_batch* h, aleph_hip_b
   s.local_qubits = b.cfg.local_qubits;
   s.chunks_per_item = b.cfg.chunks_per_item;
   s.groups_per_cu = b.cfg.groups_per_cu;
+  s.solo_instances = (uint32_t)b.solo.size();
   *out = s;
   return ALEPH_HIP_OK;
 }

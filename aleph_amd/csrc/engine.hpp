@@ -69,6 +69,8 @@ struct PlanOptions {
   bool fuse_2q = true;        // fuse 1q neighbours into 2q gates (4x4 blocks)
   bool fuse_diag = true;      // fuse {diagonal, CNOT} runs into one DiagonalPhase
   uint32_t kq_width = 0;      // >= 3: fuse gates into dense blocks of up to this many qubits (standalone sweeps)
+  bool renormalize = false;   // after the plan: FP64 norm, then psi /= sqrt(norm) on the device
+  bool generic_phase = false; // generated kernels keep phase ops as the run-time term loop
 };
 
 // One device launch of a plan.

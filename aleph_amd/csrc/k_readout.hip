@@ -401,6 +401,51 @@ __global__ void __launch_bounds__(kThreads) k_sample(const C2<T>* __restrict__ s
 
 // Capped so that a dispatch stays under 2^32 work-items; kernels that can
 // exceed the cap loop grid-stride (k_sample's grid is shots * 64 threads).
+// psi *= 1 / sqrt(norm[0] + norm[1]), the FP64 norm a preceding reduce left
+// on the device; a non-finite or zero norm leaves the state alone.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_scale_by_norm(C2<T>* __restrict__ s, uint64_t len,
+                                                             const double* __restrict__ norm) {
+  const double t = norm[0] + norm[1];
+  if (norm[2] != 0.0 || !(t > 0.0) || !isfinite(t)) return;
+  const double f = 1.0 / sqrt(t);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
+    C2<T> a = s[i];
+    a.x = (T)((double)a.x * f);
+    a.y = (T)((double)a.y * f);
+    s[i] = a;
+  }
+}
+
+// Per block: Re and Im of sum conj(a_i) b_i, sum |a_i - b_i|^2 (compensated,
+// FP64) and max |a_i - b_i|^2; a and b may differ in precision.
+template <typename TA, typename TB>
+__global__ void __launch_bounds__(kThreads) k_compare(const C2<TA>* __restrict__ a, const C2<TB>* __restrict__ b,
+                                                       uint64_t len, double* __restrict__ partials) {
+  KSum v[3] = {{0, 0}, {0, 0}, {0, 0}};
+  double mx = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
+    const double ar = a[i].x, ai = a[i].y, br = b[i].x, bi = b[i].y;
+    kadd(v[0], ar * br + ai * bi);
+    kadd(v[1], ar * bi - ai * br);
+    const double dr = ar - br, di = ai - bi, d2 = dr * dr + di * di;
+    kadd(v[2], d2);
+    mx = fmax(mx, d2);
+  }
+  block_reduce<3>(v);
+  __shared__ double shm[kThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+  if ((threadIdx.x & 63u) == 0) shm[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < blockDim.x / 64; w++) mx = fmax(mx, shm[w]);
+    partials[4 * blockIdx.x + 0] = v[0].s + v[0].c;
+    partials[4 * blockIdx.x + 1] = v[1].s + v[1].c;
+    partials[4 * blockIdx.x + 2] = v[2].s + v[2].c;
+    partials[4 * blockIdx.x + 3] = mx;
+  }
+}
+
 inline uint32_t grid_for(uint64_t items) {
   uint64_t g = (items + kThreads - 1) / kThreads;
   if (g > (1ull << 20)) g = 1ull << 20;
@@ -507,9 +552,35 @@ void launch_sample(hipStream_t st, const void* state, uint32_t n, double* block_
   ALEPH_HIP_CHECK(hipGetLastError());
 }
 
+template <typename T>
+void launch_normalize(hipStream_t st, void* state, uint32_t n, double* partials, double* res) {
+  launch_reduce_branch<T>(st, state, n, -1, partials, res);
+  const uint64_t len = 1ull << n;
+  hipLaunchKernelGGL(k_scale_by_norm<T>, dim3(grid_for(len)), dim3(kThreads), 0, st, (C2<T>*)state, len,
+                     (const double*)res);
+  ALEPH_HIP_CHECK(hipGetLastError());
+}
+
+template <typename TA, typename TB>
+uint32_t launch_compare(hipStream_t st, const void* a, const void* b, uint32_t n, double* partials) {
+  const uint64_t len = 1ull << n;
+  uint32_t blocks = grid_for(len);
+  if (blocks > kReduceBlocks) blocks = kReduceBlocks;
+  hipLaunchKernelGGL((k_compare<TA, TB>), dim3(blocks), dim3(kThreads), 0, st, (const C2<TA>*)a, (const C2<TB>*)b,
+                     len, partials);
+  ALEPH_HIP_CHECK(hipGetLastError());
+  return blocks;
+}
+
+template uint32_t launch_compare<double, double>(hipStream_t, const void*, const void*, uint32_t, double*);
+template uint32_t launch_compare<double, float>(hipStream_t, const void*, const void*, uint32_t, double*);
+template uint32_t launch_compare<float, double>(hipStream_t, const void*, const void*, uint32_t, double*);
+template uint32_t launch_compare<float, float>(hipStream_t, const void*, const void*, uint32_t, double*);
+
 #define ALEPH_INST(T)                                                                                           \
   template void launch_reduce_branch<T>(hipStream_t, const void*, uint32_t, int32_t, double*, double*);         \
   template void launch_sum_p2<T>(hipStream_t, const void*, uint32_t, double*, double*);                         \
+  template void launch_normalize<T>(hipStream_t, void*, uint32_t, double*, double*);                            \
   template void launch_expect_group<T>(hipStream_t, const void*, uint32_t, uint64_t, const uint64_t*,            \
                                        const uint8_t*, uint32_t, double*, double*);                             \
   template void launch_collapse<T>(hipStream_t, void*, uint32_t, uint32_t, int32_t, double);                    \

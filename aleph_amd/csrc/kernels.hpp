@@ -112,6 +112,13 @@ void launch_reduce_branch(hipStream_t s, const void* state, uint32_t n, int32_t 
 // out[0] = sum of |a|^4, out[1] = sum of |a|^2, out[2] = non-finite count (same partials as above).
 template <typename T>
 void launch_sum_p2(hipStream_t s, const void* state, uint32_t n, double* partials, double* out);
+// FP64 norm into res (the reduce above), then psi /= sqrt(norm) on the device (no host sync).
+template <typename T>
+void launch_normalize(hipStream_t s, void* state, uint32_t n, double* partials, double* res);
+// Per-block partials (4 doubles each: Re, Im of sum conj(a) b, sum |a-b|^2, max |a-b|^2) of two
+// states of equal size; returns the block count (<= kReduceBlocks).
+template <typename TA, typename TB>
+uint32_t launch_compare(hipStream_t s, const void* a, const void* b, uint32_t n, double* partials);
 // Up to kExpectGroup Pauli terms sharing X mask `xmask`, one read of the state:
 // out[t] = sum_i (-1)^{|i & zy[t]|} w_t(i), with w_t = |psi_i|^2 when xmask == 0 and
 // otherwise Re (use_im[t] == 0) or Im of conj(psi[i ^ xmask]) psi[i] over the

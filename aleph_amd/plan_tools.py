@@ -65,7 +65,7 @@ class HostPlan:
 
     def __init__(self, circuit, precision: int = 64, planner: int = L.PLANNER_RULE, local_qubits: int = 0,
                  max_ops: int = 0, coalesce_bits: int = 0, fuse_1q: bool = True, reg_bits: int = 0, nontemporal: bool = False,
-                fuse_2q: bool = True, fuse_diag: bool = True, kq_width: int = 0):
+                fuse_2q: bool = True, fuse_diag: bool = True, kq_width: int = 0, flags: int = 0):
         lib = L.load()
         words, reals = circuit.encode()
         nw, nr = int(words.size), int(reals.size)
@@ -73,7 +73,7 @@ class HostPlan:
         reals = np.ascontiguousarray(reals if nr else np.zeros(1))
         opts = L.PlanOpts(planner, local_qubits, max_ops, coalesce_bits, int(fuse_1q), reg_bits,
                           (L.PLAN_NONTEMPORAL if nontemporal else 0) | (0 if fuse_2q else L.PLAN_NO_FUSE_2Q)
-                          | (0 if fuse_diag else L.PLAN_NO_FUSE_DIAG), kq_width)
+                          | (0 if fuse_diag else L.PLAN_NO_FUSE_DIAG) | flags, kq_width)
         h = C.c_void_p()
         err = L.Error()
         code = lib.aleph_hip_plan_build_host(circuit.num_qubits, precision, words.ctypes.data_as(L.PU32), nw,

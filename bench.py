@@ -68,7 +68,9 @@ def parse():
     ap.add_argument("--traffic-child", action="store_true",
                     help="only reset, execute one plan and take the norm (the process rocprofv3 profiles)")
     ap.add_argument("--child-precision", type=int, default=64, choices=(32, 64))
-    ap.add_argument("--child-workload", default="brickwall", choices=("brickwall", "qft", "noise", "exchange"))
+    ap.add_argument("--child-workload", default="brickwall",
+                    choices=("brickwall", "qft", "noise", "exchange", "qaoa_batch"))
+    ap.add_argument("--child-kq", type=int, default=0, help="plan the child's circuit into dense blocks of this width")
     return ap.parse_args()
 
 
@@ -194,7 +196,9 @@ def traffic_child(args) -> None:
     (k_init_basis writes the whole state), one execution of the plan, one
     norm (k_reduce_branch reads the whole state).  The two full-state kernels
     calibrate the counters.  --child-precision and --child-workload pick the
-    plan (FP64 brickwall by default)."""
+    plan (FP64 brickwall by default), --child-kq plans it into dense blocks
+    (matrix-core sweeps for widths 4..6); the noise, exchange and qaoa_batch
+    workloads run those legs' kernels once instead."""
     from aleph_amd import workloads as W
     from aleph_amd.backend import HipSvBackend
     if args.child_workload == "noise":
@@ -210,9 +214,20 @@ def traffic_child(args) -> None:
         for k in (1, 2, 3):
             dx.exchange(ranks, layout, list(range(args.n - 1, args.n - 1 - k, -1)))
         return
+    if args.child_workload == "qaoa_batch":
+        from aleph_amd.batch import HipBatch
+        graphs, ga, be = W.qaoa_batch(args.batch_count, seed=args.seed)
+        bt = HipBatch.qaoa(HipSvBackend(seed=1, precision=32), graphs, ga, be, precision=32, mode=1,
+                           local_qubits=12, chunks_per_item=16)
+        bt.run()
+        bt.run()
+        bt.expect_maxcut()
+        bt.free()
+        return
     b = HipSvBackend(seed=1, precision=args.child_precision)
     c = W.qft(args.n) if args.child_workload == "qft" else W.brickwall(args.n, args.depth, seed=args.seed)
-    plan = b.compile(c, local_qubits=args.local_qubits, max_ops=args.max_ops, jit=not args.no_jit)
+    plan = b.compile(c, local_qubits=args.local_qubits, max_ops=args.max_ops, jit=not args.no_jit,
+                     kq_width=args.child_kq)
     st = b.allocate(args.n)
     plan.execute(st)
     b.norm_sqr(st)
@@ -302,7 +317,8 @@ def batch_leg(args) -> dict:
     b.free()
     return {"instances": len(graphs), "ms": ms, "instances_per_s": len(graphs) / (ms * 1e-3),
             "hbm_gbs": st["sweep_bytes"] / (ms * 1e-3) / 1e9, "sweep_bytes": st["sweep_bytes"],
-            "passes": st["n_passes"], "items": st["n_items"], "pool_gb": st["pool_bytes"] / 1e9,
+            "passes": st["n_passes"], "items": st["n_items"], "solo_instances": st["solo_instances"],
+            "pool_gb": st["pool_bytes"] / 1e9,
             "plan_ms": st["plan_ms"], "mean_cut": float(np.mean(cvals)),
             "tuned": {"mode": ["persistent", "rounds"][best["mode"]], "local_qubits": best["local_qubits"],
                       "chunks_per_item": best["chunks_per_item"]},
@@ -393,6 +409,13 @@ def main():
 
     fused = launch_ms[kinds == 0]
     sweep_bytes = 2.0 * (1 << n) * 16.0
+    # The practical in-place streaming ceiling of a fused pass on this device:
+    # one pass whose only op is a diagonal on qubit 0 (the pass skeleton,
+    # every amplitude read and written once, no arithmetic to speak of).
+    from aleph_amd.circuit import Circuit
+    _, _, s_launch, s_kinds, _, s_st = time_circuit(b64, Circuit(n).rz(0.25, 0), 1, 1, **plan_kw)
+    s_st.free()
+    stream_ms = float(np.min(s_launch[s_kinds == 0])) if np.any(s_kinds == 0) else float("nan")
     avg_ms = float(fused.mean()) if fused.size else float("nan")
     achieved = sweep_bytes / (avg_ms * 1e-3) / 1e9 if fused.size else 0.0
     total_bytes = stats["sweep_bytes"]
@@ -421,6 +444,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_fused_pass<double>" if args.no_jit else "aleph_pass_f64 (generated per pass)",
+                     "stream_ceiling": {"pass_ms": stream_ms, "gbs": sweep_bytes / (stream_ms * 1e-3) / 1e9,
+                                        "frac": sweep_bytes / (stream_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                        "what": "one fused pass with a single diagonal op at the same n: "
+                                                "the in-place read+write streaming rate of the pass skeleton"},
+                     "frac_of_stream_ceiling": (stream_ms / avg_ms) if fused.size else None,
                      "avg_launch_ms": avg_ms, "launches": int(fused.size),
                      "algorithmic_bytes_per_launch": sweep_bytes,
                      "fp64": {"algorithmic_flops_per_launch": flops,

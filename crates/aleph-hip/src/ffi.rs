@@ -75,6 +75,8 @@ pub const ALEPH_HIP_PLAN_NONTEMPORAL: u32 = 1;
 pub const ALEPH_HIP_PLAN_NO_FUSE_2Q: u32 = 2;
 pub const ALEPH_HIP_PLAN_NO_FUSE_DIAG: u32 = 4;
 pub const ALEPH_HIP_PLAN_JIT: u32 = 8;
+pub const ALEPH_HIP_PLAN_RENORMALIZE: u32 = 16;
+pub const ALEPH_HIP_PLAN_GENERIC_PHASE: u32 = 32;
 
 pub const ALEPH_HIP_BATCH_PERSISTENT: u32 = 0;
 pub const ALEPH_HIP_BATCH_ROUNDS: u32 = 1;
@@ -173,7 +175,7 @@ pub struct aleph_hip_batch_opts {
     pub max_ops_per_pass: u32,
     pub spin_limit: u32,
     pub flags: u32,
-    pub reserved: u32,
+    pub solo_qubits: u32,
 }
 
 #[repr(C)]
@@ -184,7 +186,7 @@ pub struct aleph_hip_batch_stats {
     pub pool_bytes: u64,
     pub n_instances: u32, pub n_passes: u32, pub n_items: u32, pub n_rounds: u32,
     pub grid: u32, pub threads: u32, pub lds_bytes: u32, pub mode: u32,
-    pub local_qubits: u32, pub chunks_per_item: u32, pub groups_per_cu: u32, pub reserved: u32,
+    pub local_qubits: u32, pub chunks_per_item: u32, pub groups_per_cu: u32, pub solo_instances: u32,
 }
 
 #[repr(C)]
@@ -290,6 +292,11 @@ extern "C" {
     pub fn aleph_hip_batch_build_host_qaoa(n_instances: u32, num_qubits: *const u32, edge_offsets: *const u64, edges: *const u32, p: u32, gammas: *const f64, betas: *const f64, precision: u32, opts: *const aleph_hip_batch_opts, out: *mut *mut aleph_hip_batch, err: *mut aleph_hip_error) -> aleph_hip_status;
     pub fn aleph_hip_batch_view_get(b: *const aleph_hip_batch, out: *mut aleph_hip_batch_view) -> aleph_hip_status;
 
+    pub fn aleph_hip_state_normalize(ctx: *mut aleph_hip_ctx, st: *mut aleph_hip_state, norm_before: *mut f64)
+        -> aleph_hip_status;
+    pub fn aleph_hip_state_compare(
+        ctx: *mut aleph_hip_ctx, a: *mut aleph_hip_state, b: *mut aleph_hip_state, out: *mut f64
+    ) -> aleph_hip_status;
     pub fn aleph_hip_dist_alloc_rank(
         ctx: *mut aleph_hip_ctx, m: u32, precision: u32, rank: u32, out: *mut *mut aleph_hip_state
     ) -> aleph_hip_status;

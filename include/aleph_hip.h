@@ -186,7 +186,12 @@ enum {
   ALEPH_HIP_PLAN_NONTEMPORAL = 1, /* stream chunks with non-temporal loads/stores */
   ALEPH_HIP_PLAN_NO_FUSE_2Q = 2,  /* keep 1q neighbours out of 2q gates (no 4x4 blocks) */
   ALEPH_HIP_PLAN_NO_FUSE_DIAG = 4, /* keep {diagonal, CNOT} runs gate by gate (no DiagonalPhase fusion) */
-  ALEPH_HIP_PLAN_JIT = 8          /* fused passes run as kernels generated per pass (hiprtc at plan creation) */
+  ALEPH_HIP_PLAN_JIT = 8,         /* fused passes run as kernels generated per pass (hiprtc at plan creation) */
+  ALEPH_HIP_PLAN_RENORMALIZE = 16,/* after every execution: FP64 norm, then psi /= sqrt(norm), on the device
+                                     (FP32 states drift ~1e-8 per pass; see aleph_hip_state_normalize) */
+  ALEPH_HIP_PLAN_GENERIC_PHASE = 32 /* generated kernels keep phase ops as the run-time term loop, so that
+                                     circuits differing only in their phase terms (QAOA instances of one
+                                     size) share kernels */
 };
 
 typedef struct aleph_hip_plan_stats {
@@ -282,6 +287,13 @@ aleph_hip_status aleph_hip_expectation_sum(aleph_hip_ctx* ctx, aleph_hip_state* 
 /* Linear cross-entropy benchmark of the exact state, 2^n * sum_x |psi_x|^4 - 1, accumulated in
  * FP64 in one read of the state: benches/src/lib.rs:29 linear_xeb (computed there on the host
  * from the amplitudes). */
+/* FP32 path renormalisation (the FP64 accumulation of crates/aleph-cuda/src/sv/readout_f32.rs:1-10
+ * applied to the norm): psi /= sqrt(sum |psi|^2) with the sum in FP64 on the device; the norm
+ * before scaling goes to *norm_before when it is not NULL (then the call synchronises). */
+aleph_hip_status aleph_hip_state_normalize(aleph_hip_ctx* ctx, aleph_hip_state* st, double* norm_before);
+/* Two states of one size, any precisions: out = {Re <a|b>, Im <a|b>, sum |a-b|^2, max |a-b|},
+ * accumulated in FP64 on the device (fidelity |<a|b>|^2 of normalised states). */
+aleph_hip_status aleph_hip_state_compare(aleph_hip_ctx* ctx, aleph_hip_state* a, aleph_hip_state* b, double* out);
 aleph_hip_status aleph_hip_linear_xeb(aleph_hip_ctx* ctx, aleph_hip_state* st, double* out);
 /* Marginal over `qubits` (qubits[0] = LSB of the output index), 2^n_qubits doubles. */
 aleph_hip_status aleph_hip_probabilities(aleph_hip_ctx* ctx, aleph_hip_state* st, const uint32_t* qubits,
@@ -349,7 +361,8 @@ typedef struct aleph_hip_batch_opts {
   uint32_t max_ops_per_pass; /* planner budget (0 = default) */
   uint32_t spin_limit;       /* bound on a dependency wait, in polls (0 = default) */
   uint32_t flags;            /* ALEPH_HIP_BATCH_NONTEMPORAL */
-  uint32_t reserved;
+  uint32_t solo_qubits;      /* instances of at least this many qubits run as their own plans with
+                                per-pass generated kernels after the packed ones (0 = 21, > 40 = never) */
 } aleph_hip_batch_opts;
 typedef struct aleph_hip_batch_stats {
   double plan_ms;
@@ -357,7 +370,7 @@ typedef struct aleph_hip_batch_stats {
   uint64_t pool_bytes;
   uint32_t n_instances, n_passes, n_items, n_rounds;
   uint32_t grid, threads, lds_bytes, mode;
-  uint32_t local_qubits, chunks_per_item, groups_per_cu, reserved;
+  uint32_t local_qubits, chunks_per_item, groups_per_cu, solo_instances;
 } aleph_hip_batch_stats;
 typedef struct aleph_hip_batch_tune {
   uint32_t mode, local_qubits, chunks_per_item, groups_per_cu;

@@ -213,3 +213,40 @@ def test_rounds_replay_after_a_zero_angle_changes_the_plan(hip, orc):
         b.run()
         check_qaoa_batch(orc, b, graphs, gammas, betas, PARITY_TOL)
     b.free()
+
+
+@pytest.mark.parametrize("mode", [L.BATCH_PERSISTENT, L.BATCH_ROUNDS])
+@pytest.mark.parametrize("precision", [64, 32])
+def test_solo_instances_match_oracle(hip, orc, mode, precision):
+    """Instances at or above solo_qubits run as their own plans (generated
+    kernels) after the packed ones; every instance must still equal the
+    oracle, with graph replays across angle updates."""
+    graphs, gammas, betas = small_qaoa(12, seed=31 + mode + precision, n_min=8, n_max=14, p=2)
+    b = HipBatch.qaoa(hip, graphs, gammas, betas, precision=precision, mode=mode, local_qubits=8,
+                      solo_qubits=12)
+    st = b.stats()
+    assert st["solo_instances"] == sum(1 for n, _ in graphs if n >= 12) > 0
+    tol = PARITY_TOL if precision == 64 else FP32_TOL
+    rng = np.random.default_rng(1)
+    for rep in range(2):
+        if rep:
+            gammas = rng.uniform(0, np.pi, size=gammas.shape)
+            betas = rng.uniform(0, np.pi, size=betas.shape)
+            b.set_qaoa_angles(gammas, betas)
+        b.run()
+        check_qaoa_batch(orc, b, graphs, gammas, betas, tol)
+        b.run()
+        check_qaoa_batch(orc, b, graphs, gammas, betas, tol)
+    b.free()
+
+
+def test_all_solo_batch_and_default_threshold(hip, orc):
+    graphs, gammas, betas = small_qaoa(4, seed=41, n_min=10, n_max=12, p=2)
+    b = HipBatch.qaoa(hip, graphs, gammas, betas, precision=64, solo_qubits=5)
+    assert b.stats()["solo_instances"] == 4 and b.stats()["n_items"] == 0
+    b.run()
+    check_qaoa_batch(orc, b, graphs, gammas, betas, PARITY_TOL)
+    b.free()
+    b = HipBatch.qaoa(hip, graphs, gammas, betas, precision=64)
+    assert b.stats()["solo_instances"] == 0      # default threshold: 21 qubits
+    b.free()

@@ -97,26 +97,32 @@ def test_learned_without_model_falls_back_to_rule():
 
 
 @pytest.mark.skipif(not DEFAULT_PATH.exists(), reason="no trained cost model in the package")
-def test_trained_model_does_not_lose_to_rule_planner():
-    """Held-out circuits (a seed the training corpus did not use): geometric
-    mean of measured rule time / learned time must be >= 0.97, and the
-    figure is printed."""
+def test_trained_model_beats_rule_planner():
+    """Held-out circuits (a corpus seed training never used, n 24-26):
+    geometric mean of measured rule time / learned time must exceed 1.0.
+    Where the learned planner keeps the rule planner's own options the plan
+    is the same plan, so its ratio is 1 by construction; every other
+    circuit is timed both ways (best of three).  The figures are printed."""
     b = HipSvBackend(seed=2, precision=64)
     m = CostModel.load(b)
     m.install()
-    jit = m.meta.get("codegen") == "generated"    # time the code path the model was trained on
-    specs = W.planner_corpus(8, 999, n_min=22, n_max=24, d_max=80)
+    specs = W.planner_corpus(8, 999, n_min=24, n_max=26, d_max=60)
     ratios = []
     states = {}
     for spec in specs:
         c = W.corpus_circuit(spec)
         n = c.num_qubits
         states.setdefault(n, b.allocate(n))
-        _, _, rule_ms = record(b, c, states[n], reps=3, jit=jit)
-        plan = b.compile(c, planner=L.PLANNER_LEARNED, jit=jit)
+        plan = b.compile(c, planner=L.PLANNER_LEARNED, jit=True)
+        st = plan.stats()
+        if st["chosen_rule"]:
+            plan.free()
+            ratios.append(1.0)
+            continue
         learned_ms = min(plan.execute_timed(states[n])[0].sum() for _ in range(3))
         plan.free()
+        _, _, rule_ms = record(b, c, states[n], reps=3, jit=True)
         ratios.append(rule_ms / learned_ms)
     gain = math.exp(np.mean(np.log(ratios)))
     print(f"learned vs rule on held-out circuits: geometric-mean speedup {gain:.3f} ({np.round(ratios, 3)})")
-    assert gain >= 0.97
+    assert gain > 1.0

@@ -365,3 +365,46 @@ def test_fp32_brickwall(hip32, orc, n):
     print(f"fp32 brickwall n={n} max_abs={err:.3e} 1-F={1 - fid:.3e}", flush=True)
     assert err < 1e-5
     assert 1 - fid < 1e-6
+
+
+def test_normalize_and_compare(hip):
+    c = W.brickwall(14, 8, seed=3)
+    a = hip.allocate(14)
+    hip.run_program(a, c, jit=True)
+    b = hip.allocate(14, precision=32)            # one context, two precisions
+    hip.run_program(b, c, jit=True)
+    r = hip.compare(a, a)
+    assert abs(r["fidelity"] - 1.0) < 1e-12 and r["max_abs"] == 0.0 and r["l2_sq"] == 0.0
+    r = hip.compare(a, b)          # FP64 against FP32, accumulated in FP64
+    assert 1.0 - r["fidelity"] < 1e-6 and r["max_abs"] < 1e-5
+    want = a.amplitudes()
+    assert abs(r["overlap"] - np.vdot(want, b.amplitudes())) < 1e-9
+    b.load(1.5 * b.amplitudes())
+    assert abs(hip.normalize(b) - 2.25) < 1e-5
+    assert abs(hip.norm_sqr(b) - 1.0) < 1e-6
+    assert max_abs(b.amplitudes(), want) < 1e-5
+    a.free()
+    b.free()
+
+
+def test_renormalized_and_generic_phase_plans(hip, hip32):
+    """RENORMALIZE ends every execution at norm 1 in FP64; GENERIC_PHASE
+    kernels equal the per-class generated ones."""
+    edges = W.random_3_regular(16, 7)
+    c = W.qaoa_maxcut(16, edges, [0.4, 1.1, 0.7], [0.9, 0.2, 1.3])
+    outs = []
+    for generic in (False, True):
+        st = hip.allocate(16)
+        p = hip.compile(c, jit=True, generic_phase=generic)
+        p.execute(st)
+        outs.append(st.amplitudes())
+        p.free()
+        st.free()
+    assert max_abs(outs[0], outs[1]) < PARITY_TOL
+    st = hip32.allocate(18)
+    p = hip32.compile(W.brickwall(18, 40, seed=5), jit=True, renormalize=True)
+    for _ in range(3):
+        p.execute(st)
+    assert abs(hip32.norm_sqr(st) - 1.0) < 2e-7
+    p.free()
+    st.free()
