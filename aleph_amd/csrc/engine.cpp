@@ -1156,7 +1156,38 @@ std::vector<ProgInstr> fuse_dense_blocks(const std::vector<ProgInstr>& prog, uin
   return out;
 }
 
-std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog, uint32_t n, uint32_t precision,
+namespace {
+// A run of uncontrolled SWAPs (and barriers) that ends the program is a
+// qubit permutation sigma (sigma[q]: the position bit q's value ends at).
+// When it is an involution with at least two SWAPs (disjoint transpositions,
+// the QFT's final reversal among them) it runs as one in-place sweep instead
+// of swap passes: returns where the run starts, else prog.size().
+size_t trailing_swap_involution(const std::vector<ProgInstr>& prog, uint32_t n, std::vector<uint32_t>& sigma) {
+  size_t cut = prog.size(), swaps = 0;
+  while (cut > 0) {
+    const ProgInstr& pi = prog[cut - 1];
+    const bool swap = pi.type == ProgInstr::GATE && pi.gate.kind == ALEPH_HIP_GATE_SWAP && pi.gate.controls.empty();
+    if (pi.type != ProgInstr::BARRIER && !swap) break;
+    swaps += swap ? 1 : 0;
+    cut--;
+  }
+  sigma.resize(n);
+  for (uint32_t q = 0; q < n; q++) sigma[q] = q;
+  for (size_t i = cut; i < prog.size(); i++) {
+    if (prog[i].type != ProgInstr::GATE) continue;
+    const uint32_t a = prog[i].gate.qubits[0], b = prog[i].gate.qubits[1];
+    for (uint32_t& v : sigma) v = v == a ? b : (v == b ? a : v);
+  }
+  bool identity = true, involution = true;
+  for (uint32_t q = 0; q < n; q++) {
+    identity &= sigma[q] == q;
+    involution &= sigma[sigma[q]] == q;
+  }
+  return swaps >= 2 && involution && !identity ? cut : prog.size();
+}
+}  // namespace
+
+std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog_in, uint32_t n, uint32_t precision,
                                  const PlanOptions& opts) {
   const auto t0 = std::chrono::steady_clock::now();
   auto p = std::make_unique<Plan>();
@@ -1164,17 +1195,27 @@ std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog, uint32_t n,
   p->precision = precision;
   p->opts = opts;
   uint32_t gates = 0;
-  for (const ProgInstr& pi : prog) gates += pi.type == ProgInstr::GATE ? 1 : 0;
+  for (const ProgInstr& pi : prog_in) gates += pi.type == ProgInstr::GATE ? 1 : 0;
   const uint32_t G = opts.reg_bits ? opts.reg_bits : (uint32_t)kRegBits;
   if (opts.planner == ALEPH_HIP_PLANNER_UNFUSED || n < G) {
-    plan_unfused(*p, prog);
+    plan_unfused(*p, prog_in);
   } else {
+    std::vector<uint32_t> sigma;
+    const size_t cut = trailing_swap_involution(prog_in, n, sigma);
+    const std::vector<ProgInstr> prog(prog_in.begin(), prog_in.begin() + (ptrdiff_t)cut);
     std::vector<ProgInstr> pre = opts.fuse_diag ? fuse_diagonal_runs(prog, n) : prog;
     if (opts.kq_width >= 3) pre = fuse_dense_blocks(pre, std::min<uint32_t>(opts.kq_width, kMaxFusedK));
     std::vector<LGate> lg = lower_program(pre, n);
     if (opts.fuse_1q) lg = fuse_1q_runs(lg);
     if (opts.fuse_2q) lg = fuse_pairs(lg);
     plan_rule(*p, std::move(lg), n, opts);
+    if (cut < prog_in.size()) {
+      Step st;
+      st.kind = Step::BITPERM;
+      st.count = n;
+      st.dev_off = push_raw(p->payload, sigma);
+      p->steps.push_back(st);
+    }
   }
   const auto t1 = std::chrono::steady_clock::now();
   aleph_hip_plan_stats& st = p->stats;
@@ -1318,6 +1359,9 @@ static void execute_t(const Plan& p, hipStream_t s, void* state, hipEvent_t* ev)
         break;
       case Step::SWAPQ:
         launch_swap<T>(s, state, p.n, st.qubits[0], st.qubits[1], st.cmask);
+        break;
+      case Step::BITPERM:
+        launch_bitperm<T>(s, state, p.n, reinterpret_cast<const uint32_t*>(p.payload.data() + st.dev_off));
         break;
     }
   }

@@ -76,7 +76,7 @@ struct PlanOptions {
 // One device launch of a plan.
 // Standard-layout so the host plan view (aleph_hip_plan_view_get) can expose it.
 struct Step {
-  enum Kind : uint32_t { FUSED, KQ, DIAG, POLY, G1Q, G2Q, SWAPQ };
+  enum Kind : uint32_t { FUSED, KQ, DIAG, POLY, G1Q, G2Q, SWAPQ, BITPERM };
   uint32_t kind = FUSED;
   uint32_t pass = 0;          // FUSED: index into passes
   uint32_t m = 0, n_stages = 0;

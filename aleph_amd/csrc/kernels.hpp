@@ -41,6 +41,11 @@ template <typename T>
 void launch_permute(hipStream_t s, const void* src, void* dst, uint32_t n, const uint32_t* perm);
 void launch_widen_f32(hipStream_t s, const void* src_f32, void* dst_f64, uint64_t len);
 
+// ---- qubit permutations that are involutions, in place in one sweep (k_exchange.hip) ----
+// sigma[q] = the qubit position bit q moves to; sigma o sigma = id.
+template <typename T>
+void launch_bitperm(hipStream_t s, void* state, uint32_t n, const uint32_t* sigma);
+
 // ---- rank-slice exchange of a distributed state on one device (k_exchange.hip) ----
 constexpr uint32_t kMaxExchangeBits = 16;
 struct ExchangeBits {

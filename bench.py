@@ -404,8 +404,20 @@ def main():
     b64 = HipSvBackend(seed=1)
     plan_kw = dict(local_qubits=args.local_qubits, max_ops=args.max_ops, jit=not args.no_jit)
     ms, stats, launch_ms, kinds, norm, st = time_circuit(b64, circuit, args.steps, args.warmup, **plan_kw)
-    st.free()
     log(f"bench: fp64 {ms:.2f} ms/circuit passes={stats['n_passes']} norm-1={norm - 1:.2e}")
+    fp32_vs_fp64 = None
+    if not args.no_extra:
+        # The FP32 run of the same plan beside the FP64 state, compared on the
+        # device (overlap and max |difference| accumulated in FP64).
+        s32 = b64.allocate(n, precision=32)
+        p32 = b64.compile(circuit, precision=32, **plan_kw)
+        p32.execute(s32)
+        cmp = b64.compare(st, s32)
+        p32.free()
+        fp32_vs_fp64 = {"one_minus_fidelity": 1.0 - cmp["fidelity"] / (norm * b64.norm_sqr(s32)),
+                        "max_abs": cmp["max_abs"]}
+        s32.free()
+    st.free()
 
     fused = launch_ms[kinds == 0]
     sweep_bytes = 2.0 * (1 << n) * 16.0
@@ -467,7 +479,11 @@ def main():
         s.free()
         log(f"bench: fp32 {ms32:.2f} ms/circuit passes={st32['n_passes']}")
         out["fp32"] = {"ms_per_step": ms32, "passes": st32["n_passes"], "speedup_vs_fp64_rule": ms / ms32,
-                       "norm_error": norm32 - 1.0}
+                       "norm_error": norm32 - 1.0, "vs_fp64_state": fp32_vs_fp64}
+        msr, _, _, _, normr, sr = time_circuit(b32, circuit, args.steps, args.warmup, renormalize=True, **plan_kw)
+        sr.free()
+        out["fp32"]["renormalized"] = {"ms_per_step": msr, "norm_error": normr - 1.0,
+                                       "what": "the same plan with the FP64-normed renormalisation after it"}
         if DEFAULT_PATH.exists():
             for prec, bk in ((32, b32), (64, b64)):
                 model = CostModel.load(bk)

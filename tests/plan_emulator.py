@@ -114,6 +114,14 @@ def emulate(plan: HostPlan, psi: np.ndarray, precision: int = 64) -> np.ndarray:
             m = np.array(st.mat8[:2 * d * d]).view(np.complex128).reshape(d, d)
             qubits = [st.qubits[0]] if d == 2 else [st.qubits[0], st.qubits[1]]
             psi = _apply_matrix(psi, n, qubits, m, int(st.cmask))
+        elif st.kind == 7:      # involutive qubit permutation: bit q moves to sigma[q]
+            sigma = np.frombuffer(plan.payload, dtype=np.uint32, count=n, offset=st.dev_off)
+            dst = np.zeros(1 << n, dtype=np.int64)
+            for q in range(n):
+                dst |= ((idx >> q) & 1) << int(sigma[q])
+            out = np.empty_like(psi)
+            out[dst] = psi
+            psi = out
         else:
             raise AssertionError(f"unknown step kind {st.kind}")
     return psi

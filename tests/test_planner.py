@@ -196,3 +196,32 @@ def test_dense_block_count_falls_with_width():
     c = W.brickwall(16, 20, seed=1)
     counts = [sum(1 for s in HostPlan(c, kq_width=w).steps if s.kind == 1) for w in (3, 4, 5, 6)]
     assert counts[0] > counts[1] > counts[2] >= counts[3], counts
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_trailing_swap_involution_is_one_permutation_step(oracle_lib, seed):
+    """A program ending in disjoint SWAPs plans them as one in-place qubit
+    permutation (step kind 7); anything else keeps the swap passes."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(6, 13))
+    c = W.brickwall(n, 3, seed=seed)
+    perm = rng.permutation(n)
+    pairs = [(int(perm[2 * i]), int(perm[2 * i + 1])) for i in range(int(rng.integers(2, n // 2 + 1)))]
+    for a, b in pairs:
+        c.swap(a, b)
+    c.barrier(list(range(n)))
+    psi = W.gaussian_state(n, 10 + seed)
+    plan = HostPlan(c)
+    assert [s.kind for s in plan.steps].count(7) == 1 and plan.steps[-1].kind == 7
+    assert max_abs(emulate(plan, psi), oracle_state(oracle_lib, c, psi)) < 1e-12
+    # a 3-cycle of swaps is no involution: no permutation step
+    c3 = W.brickwall(n, 2, seed=seed).swap(0, 1).swap(1, 2)
+    plan3 = HostPlan(c3)
+    assert 7 not in [s.kind for s in plan3.steps]
+    assert max_abs(emulate(plan3, psi), oracle_state(oracle_lib, c3, psi)) < 1e-12
+
+
+def test_qft_final_reversal_is_one_permutation_step():
+    for n in (10, 30):
+        kinds = [s.kind for s in HostPlan(W.qft(n)).steps]
+        assert kinds[-1] == 7 and kinds.count(7) == 1
