@@ -1185,6 +1185,29 @@ size_t trailing_swap_involution(const std::vector<ProgInstr>& prog, uint32_t n, 
   }
   return swaps >= 2 && involution && !identity ? cut : prog.size();
 }
+
+// The permutation kernel holds tiles over the low min(5, n) qubits and their
+// images under sigma in LDS; uncontrolled one-qubit gates on those qubits
+// that directly precede the permutation are applied there, in the same
+// sweep (the QFT's last Hadamard lands on such a qubit).
+size_t tile_gates_before(const std::vector<ProgInstr>& prog, size_t cut, uint32_t n, const std::vector<uint32_t>& sigma) {
+  uint64_t tile = 0;
+  for (uint32_t q = 0; q < std::min<uint32_t>(5u, n); q++) tile |= (1ull << q) | (1ull << sigma[q]);
+  size_t start = cut, gates = 0;
+  while (start > 0 && gates < kPermMaxGates) {
+    const ProgInstr& pi = prog[start - 1];
+    if (pi.type == ProgInstr::BARRIER) {
+      start--;
+      continue;
+    }
+    if (pi.type != ProgInstr::GATE || pi.gate.qubits.size() != 1 || !pi.gate.controls.empty() ||
+        pi.gate.dim != 2 || !((tile >> pi.gate.qubits[0]) & 1ull))
+      break;
+    start--;
+    gates++;
+  }
+  return start;
+}
 }  // namespace
 
 std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog_in, uint32_t n, uint32_t precision,
@@ -1200,9 +1223,12 @@ std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog_in, uint32_t
   if (opts.planner == ALEPH_HIP_PLANNER_UNFUSED || n < G) {
     plan_unfused(*p, prog_in);
   } else {
+    // Only states wider than a chunk: inside one chunk the swaps are register renames of the last pass.
     std::vector<uint32_t> sigma;
-    const size_t cut = trailing_swap_involution(prog_in, n, sigma);
-    const std::vector<ProgInstr> prog(prog_in.begin(), prog_in.begin() + (ptrdiff_t)cut);
+    const uint32_t m_chunk = std::min<uint32_t>(opts.local_qubits ? opts.local_qubits : 12u, (uint32_t)kMaxLocalBits);
+    const size_t cut = n > m_chunk ? trailing_swap_involution(prog_in, n, sigma) : prog_in.size();
+    const size_t body = cut < prog_in.size() ? tile_gates_before(prog_in, cut, n, sigma) : cut;
+    const std::vector<ProgInstr> prog(prog_in.begin(), prog_in.begin() + (ptrdiff_t)body);
     std::vector<ProgInstr> pre = opts.fuse_diag ? fuse_diagonal_runs(prog, n) : prog;
     if (opts.kq_width >= 3) pre = fuse_dense_blocks(pre, std::min<uint32_t>(opts.kq_width, kMaxFusedK));
     std::vector<LGate> lg = lower_program(pre, n);
@@ -1214,6 +1240,19 @@ std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog_in, uint32_t
       st.kind = Step::BITPERM;
       st.count = n;
       st.dev_off = push_raw(p->payload, sigma);
+      std::vector<uint32_t> gq;
+      std::vector<double> gm;
+      for (size_t i = body; i < cut; i++) {
+        if (prog_in[i].type != ProgInstr::GATE) continue;
+        gq.push_back(prog_in[i].gate.qubits[0]);
+        for (const cplx& z : prog_in[i].gate.m) {
+          gm.push_back(z.real());
+          gm.push_back(z.imag());
+        }
+      }
+      st.k = (uint32_t)gq.size();
+      st.dev_off2 = push_raw(p->payload, gq.empty() ? std::vector<uint32_t>{0} : gq);
+      st.dev_off3 = push_raw(p->payload, gm.empty() ? std::vector<double>{0.0} : gm);
       p->steps.push_back(st);
     }
   }
@@ -1361,7 +1400,9 @@ static void execute_t(const Plan& p, hipStream_t s, void* state, hipEvent_t* ev)
         launch_swap<T>(s, state, p.n, st.qubits[0], st.qubits[1], st.cmask);
         break;
       case Step::BITPERM:
-        launch_bitperm<T>(s, state, p.n, reinterpret_cast<const uint32_t*>(p.payload.data() + st.dev_off));
+        launch_bitperm<T>(s, state, p.n, reinterpret_cast<const uint32_t*>(p.payload.data() + st.dev_off), st.k,
+                          reinterpret_cast<const uint32_t*>(p.payload.data() + st.dev_off2),
+                          reinterpret_cast<const double*>(p.payload.data() + st.dev_off3));
         break;
     }
   }

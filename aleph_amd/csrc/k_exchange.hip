@@ -6,6 +6,7 @@
 // exchange: every moved amplitude is read once and written once, with no
 // scratch copy (crates/aleph-cuda/src/dist/exchange.rs:133-200 stages each
 // pair through a scratch buffer instead).
+#include <algorithm>
 #include <type_traits>
 
 #include "common.hpp"
@@ -45,9 +46,12 @@ __global__ void __launch_bounds__(256) k_exchange(T* const* ranks, uint32_t m, u
 // the low k qubits and their images (closed under sigma), so a tile holds
 // runs of 2^k consecutive amplitudes; the other qubits R index tiles, and
 // sigma maps tile r to tile sigma_R(r).  One workgroup loads tiles r and
-// sigma_R(r) (r <= sigma_R(r)) into LDS and writes each back permuted:
-// new[(r, t)] = old[(sigma_R(r), sigma_T(t))].  Every amplitude is read once
-// and written once.
+// sigma_R(r) (r <= sigma_R(r)), permutes each through LDS and writes it back
+// in the other's place: new[(r, t)] = old[(sigma_R(r), sigma_T(t))].  Every
+// amplitude is read once and written once.  LDS slot of tile element t is
+// t + t/32: the permuting read gathers lds[sigma_T(t)], and when sigma_T
+// sends the low five tile bits to the upper ones, 32 consecutive t would
+// otherwise hit LDS addresses 32 elements apart, all in one bank group.
 constexpr uint32_t kPermMaxTileBits = 10;
 struct PermMap {
   uint32_t nt, nr;
@@ -55,6 +59,10 @@ struct PermMap {
   uint32_t t_img[kPermMaxTileBits];     // tile bit that tile bit j moves to
   uint32_t r_pos[40];
   uint32_t r_img[40];
+  uint32_t r_from[40];                  // block-index bit that supplies bit j of the tile number
+  uint32_t ng;                          // one-qubit gates applied in LDS before permuting
+  uint32_t g_bit[kPermMaxGates];        // tile bit each gate acts on
+  double g_m[kPermMaxGates][8];
 };
 
 __device__ __forceinline__ uint64_t scatter_bits(uint64_t v, const uint32_t* pos, uint32_t count) {
@@ -69,31 +77,122 @@ __device__ __forceinline__ uint32_t permute_bits(uint32_t v, const uint32_t* img
   return x;
 }
 
+__device__ __forceinline__ uint32_t lds_slot(uint32_t t) { return t + (t >> 5); }
+
+// One-qubit gates on tile bits, applied to the tile held in LDS.
 template <typename T>
-__global__ void __launch_bounds__(256) k_bitperm(T* __restrict__ s, PermMap pm) {
+__device__ void tile_gates(T* v, const PermMap& pm, uint32_t tile) {
+  for (uint32_t g = 0; g < pm.ng; g++) {
+    const uint32_t bit = pm.g_bit[g];
+    const double* m = pm.g_m[g];
+    for (uint32_t h = threadIdx.x; h < tile / 2; h += blockDim.x) {
+      const uint32_t j0 = ((h >> bit) << (bit + 1)) | (h & ((1u << bit) - 1u));
+      const uint32_t i0 = lds_slot(j0), i1 = lds_slot(j0 | (1u << bit));
+      const double x0 = v[i0].x, y0 = v[i0].y, x1 = v[i1].x, y1 = v[i1].y;
+      v[i0].x = m[0] * x0 - m[1] * y0 + m[2] * x1 - m[3] * y1;
+      v[i0].y = m[0] * y0 + m[1] * x0 + m[2] * y1 + m[3] * x1;
+      v[i1].x = m[4] * x0 - m[5] * y0 + m[6] * x1 - m[7] * y1;
+      v[i1].y = m[4] * y0 + m[5] * x0 + m[6] * y1 + m[7] * x1;
+    }
+    __syncthreads();
+  }
+}
+
+// Both tiles are loaded into registers at once, kPermPerThread elements a
+// thread (eight: at n=30 a sweep takes 8.0 ms in FP64 against 8.2 ms at
+// four, 12 ms at two and 10 ms at sixteen, where registers limit the waves);
+// they then pass through one LDS tile in turn, so a workgroup holds a single
+// tile of LDS.
+constexpr uint32_t kPermPerThread = 8;
+
+template <typename T>
+__global__ void __launch_bounds__(128) k_bitperm(T* __restrict__ s, PermMap pm) {
   extern __shared__ unsigned char lds_raw[];
-  T* a = reinterpret_cast<T*>(lds_raw);
+  T* lds = reinterpret_cast<T*>(lds_raw);
   const uint32_t tile = 1u << pm.nt;
-  T* b = a + tile;
-  const uint32_t r = blockIdx.x;
-  uint32_t r2 = 0;
+  uint32_t r = 0, r2 = 0;
+  for (uint32_t j = 0; j < pm.nr; j++) r |= ((blockIdx.x >> pm.r_from[j]) & 1u) << j;
   for (uint32_t j = 0; j < pm.nr; j++) r2 |= ((r >> j) & 1u) << pm.r_img[j];
   if (r2 < r) return;
+  const bool pair = r2 != r;
   const uint64_t base_a = scatter_bits(r, pm.r_pos, pm.nr), base_b = scatter_bits(r2, pm.r_pos, pm.nr);
-  for (uint32_t t = threadIdx.x; t < tile; t += blockDim.x) {
-    const uint64_t off = scatter_bits(t, pm.t_pos, pm.nt);
-    a[t] = s[base_a | off];
-    if (r2 != r) b[t] = s[base_b | off];
+  // Components in plain arrays: arrays of the vector type stay in scratch
+  // memory for FP64 instead of registers.
+  using E = decltype(T::x);
+  E ax[kPermPerThread], ay[kPermPerThread], bx[kPermPerThread], by[kPermPerThread];
+#pragma unroll
+  for (uint32_t i = 0; i < kPermPerThread; i++) {
+    const uint32_t t = threadIdx.x + i * blockDim.x;
+    if (t < tile) {
+      const uint64_t off = scatter_bits(t, pm.t_pos, pm.nt);
+      const T u = s[base_a | off];
+      ax[i] = u.x;
+      ay[i] = u.y;
+      if (pair) {
+        const T w = s[base_b | off];
+        bx[i] = w.x;
+        by[i] = w.y;
+      }
+    }
+  }
+  // Tile r through LDS: its permuted values belong to tile r2.
+#pragma unroll
+  for (uint32_t i = 0; i < kPermPerThread; i++) {
+    const uint32_t t = threadIdx.x + i * blockDim.x;
+    if (t < tile) {
+      lds[lds_slot(t)].x = ax[i];
+      lds[lds_slot(t)].y = ay[i];
+    }
   }
   __syncthreads();
-  for (uint32_t t = threadIdx.x; t < tile; t += blockDim.x) {
-    const uint64_t off = scatter_bits(t, pm.t_pos, pm.nt);
-    const uint32_t src = permute_bits(t, pm.t_img, pm.nt);
-    if (r2 == r) {
-      s[base_a | off] = a[src];
-    } else {
-      s[base_a | off] = b[src];
-      s[base_b | off] = a[src];
+  tile_gates(lds, pm, tile);
+#pragma unroll
+  for (uint32_t i = 0; i < kPermPerThread; i++) {
+    const uint32_t t = threadIdx.x + i * blockDim.x;
+    if (t < tile) {
+      const T u = lds[lds_slot(permute_bits(t, pm.t_img, pm.nt))];
+      ax[i] = u.x;
+      ay[i] = u.y;
+    }
+  }
+  if (pair) {
+    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < kPermPerThread; i++) {
+      const uint32_t t = threadIdx.x + i * blockDim.x;
+      if (t < tile) {
+        lds[lds_slot(t)].x = bx[i];
+        lds[lds_slot(t)].y = by[i];
+      }
+    }
+    __syncthreads();
+    tile_gates(lds, pm, tile);
+#pragma unroll
+    for (uint32_t i = 0; i < kPermPerThread; i++) {
+      const uint32_t t = threadIdx.x + i * blockDim.x;
+      if (t < tile) {
+        const T u = lds[lds_slot(permute_bits(t, pm.t_img, pm.nt))];
+        bx[i] = u.x;
+        by[i] = u.y;
+      }
+    }
+  }
+#pragma unroll
+  for (uint32_t i = 0; i < kPermPerThread; i++) {
+    const uint32_t t = threadIdx.x + i * blockDim.x;
+    if (t < tile) {
+      const uint64_t off = scatter_bits(t, pm.t_pos, pm.nt);
+      T u, w;
+      u.x = ax[i];
+      u.y = ay[i];
+      if (pair) {
+        w.x = bx[i];
+        w.y = by[i];
+        s[base_a | off] = w;
+        s[base_b | off] = u;
+      } else {
+        s[base_a | off] = u;
+      }
     }
   }
 }
@@ -101,7 +200,8 @@ __global__ void __launch_bounds__(256) k_bitperm(T* __restrict__ s, PermMap pm) 
 }  // namespace
 
 template <typename T>
-void launch_bitperm(hipStream_t s, void* state, uint32_t n, const uint32_t* sigma) {
+void launch_bitperm(hipStream_t s, void* state, uint32_t n, const uint32_t* sigma, uint32_t n_gates,
+                    const uint32_t* gate_qubits, const double* gate_mats) {
   PermMap pm{};
   const uint32_t k = std::min<uint32_t>(5, n);
   uint64_t tmask = 0;
@@ -119,14 +219,37 @@ void launch_bitperm(hipStream_t s, void* state, uint32_t n, const uint32_t* sigm
   };
   for (uint32_t j = 0; j < pm.nt; j++) pm.t_img[j] = index_of(pm.t_pos, pm.nt, sigma[pm.t_pos[j]]);
   for (uint32_t j = 0; j < pm.nr; j++) pm.r_img[j] = index_of(pm.r_pos, pm.nr, sigma[pm.r_pos[j]]);
+  // Workgroups that run together take consecutive block indices, so the low
+  // block-index bits go to the tile-number bits whose higher qubit, over the
+  // bit and its image, is lowest: both tiles of neighbouring workgroups then
+  // differ only in low address bits (the same pages), for the QFT's reversal
+  // the middle qubits first.  With r taken straight from the block index the
+  // partner tiles of neighbouring workgroups lie 2^(n-6) amplitudes apart.
+  uint32_t order[40];
+  for (uint32_t j = 0; j < pm.nr; j++) order[j] = j;
+  std::stable_sort(order, order + pm.nr, [&](uint32_t x, uint32_t y) {
+    return std::max(pm.r_pos[x], sigma[pm.r_pos[x]]) < std::max(pm.r_pos[y], sigma[pm.r_pos[y]]);
+  });
+  for (uint32_t b = 0; b < pm.nr; b++) pm.r_from[order[b]] = b;
+  if (n_gates > kPermMaxGates) throw Failure{ALEPH_HIP_ERR_INVALID_ARGUMENT, "bitperm: too many gates"};
+  pm.ng = n_gates;
+  for (uint32_t g = 0; g < n_gates; g++) {
+    pm.g_bit[g] = index_of(pm.t_pos, pm.nt, gate_qubits[g]);
+    if (pm.g_bit[g] >= pm.nt) throw Failure{ALEPH_HIP_ERR_INVALID_ARGUMENT, "bitperm: gate off the tile"};
+    for (int e = 0; e < 8; e++) pm.g_m[g][e] = gate_mats[8 * g + e];
+  }
   using V = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
-  const size_t lds = 2 * ((size_t)1 << pm.nt) * sizeof(V);
-  hipLaunchKernelGGL(k_bitperm<V>, dim3(1u << pm.nr), dim3(256), lds, s, (V*)state, pm);
+  const size_t tile = (size_t)1 << pm.nt;
+  const size_t lds = (tile + tile / 32 + 1) * sizeof(V);
+  const uint32_t threads = std::max<uint32_t>(64, (uint32_t)(tile / kPermPerThread));
+  hipLaunchKernelGGL(k_bitperm<V>, dim3(1u << pm.nr), dim3(threads), lds, s, (V*)state, pm);
   ALEPH_HIP_CHECK(hipGetLastError());
 }
 
-template void launch_bitperm<double>(hipStream_t, void*, uint32_t, const uint32_t*);
-template void launch_bitperm<float>(hipStream_t, void*, uint32_t, const uint32_t*);
+template void launch_bitperm<double>(hipStream_t, void*, uint32_t, const uint32_t*, uint32_t, const uint32_t*,
+                                    const double*);
+template void launch_bitperm<float>(hipStream_t, void*, uint32_t, const uint32_t*, uint32_t, const uint32_t*,
+                                   const double*);
 
 template <typename T>
 void launch_exchange(hipStream_t s, void* const* d_ranks, uint32_t n_ranks, uint32_t m, uint32_t k,

@@ -42,9 +42,13 @@ void launch_permute(hipStream_t s, const void* src, void* dst, uint32_t n, const
 void launch_widen_f32(hipStream_t s, const void* src_f32, void* dst_f64, uint64_t len);
 
 // ---- qubit permutations that are involutions, in place in one sweep (k_exchange.hip) ----
-// sigma[q] = the qubit position bit q moves to; sigma o sigma = id.
+// sigma[q] = the qubit position bit q moves to; sigma o sigma = id.  Before permuting, the
+// n_gates 2x2 matrices (8 doubles each, row-major interleaved) act in order on qubits
+// gate_qubits[], which must lie among the low min(5, n) qubits and their images.
+constexpr uint32_t kPermMaxGates = 8;
 template <typename T>
-void launch_bitperm(hipStream_t s, void* state, uint32_t n, const uint32_t* sigma);
+void launch_bitperm(hipStream_t s, void* state, uint32_t n, const uint32_t* sigma, uint32_t n_gates = 0,
+                    const uint32_t* gate_qubits = nullptr, const double* gate_mats = nullptr);
 
 // ---- rank-slice exchange of a distributed state on one device (k_exchange.hip) ----
 constexpr uint32_t kMaxExchangeBits = 16;

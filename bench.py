@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--no-noise", action="store_true", help="skip the noise-trajectory leg")
     ap.add_argument("--no-dist", action="store_true", help="skip the rank-slice exchange leg")
     ap.add_argument("--noise-shots", type=int, default=100_000)
+    ap.add_argument("--batch-probe", action="store_true",
+                    help="only time each size class of the QAOA batch alone, per executor (one JSON line each)")
     ap.add_argument("--no-jit", action="store_true",
                     help="run fused passes with the compiled op-loop kernel instead of per-pass generated kernels")
     ap.add_argument("--traffic-child", action="store_true",
@@ -386,8 +388,37 @@ def exchange_leg(args) -> dict:
             "note": "host-timed synchronous call (kernel + pointer upload); bytes = moved amplitudes read + written"}
 
 
+def batch_probe(args) -> None:
+    """Each size class of the QAOA batch workload alone, per executor and
+    with or without solo instances: one JSON line per run."""
+    from aleph_amd import _lib as L
+    from aleph_amd import workloads as W
+    from aleph_amd.backend import HipSvBackend
+    from aleph_amd.batch import HipBatch
+    hip = HipSvBackend(seed=1, precision=32)
+    graphs, ga, be = W.qaoa_batch(args.batch_count, seed=args.seed)
+    ns = np.array([g[0] for g in graphs])
+    for n in sorted(set(ns.tolist())):
+        idx = np.flatnonzero(ns == n)
+        sub = [graphs[i] for i in idx]
+        for mode in (L.BATCH_PERSISTENT, L.BATCH_ROUNDS):
+            for solo in (64, 0):
+                b = HipBatch.qaoa(hip, sub, ga[idx], be[idx], precision=32, mode=mode, solo_qubits=solo)
+                b.run()
+                ms = min(b.run_timed() for _ in range(3))
+                st = b.stats()
+                b.free()
+                print(json.dumps({"n": n, "instances": len(idx), "mode": ["persistent", "rounds"][mode],
+                                  "solo_instances": st["solo_instances"], "ms": round(ms, 3),
+                                  "tbs": round(st["sweep_bytes"] / ms / 1e9, 3), "passes": st["n_passes"]}),
+                      flush=True)
+
+
 def main():
     args = parse()
+    if args.batch_probe:
+        batch_probe(args)
+        return
     if args.traffic_child:
         traffic_child(args)
         return
@@ -495,7 +526,12 @@ def main():
                 out[f"fp{prec}_learned"] = {
                     "ms_per_step": msl, "passes": stl["n_passes"], "speedup_vs_fp64_rule": ms / msl,
                     "chosen": {"local_qubits": stl["chosen_local_qubits"], "max_ops": stl["chosen_max_ops"],
-                               "fuse_2q": stl["chosen_flags"] == 0},
+                               "fuse_2q": not stl["chosen_flags"] & L.PLAN_NO_FUSE_2Q,
+                               "nontemporal": bool(stl["chosen_flags"] & L.PLAN_NONTEMPORAL),
+                               "generated_kernels": bool(stl["chosen_flags"] & L.PLAN_JIT),
+                               "kq_width": stl["chosen_kq_width"]},
+                    "kept_rule": bool(stl["chosen_rule"]),
+                    "rule_predicted_ms": stl["rule_predicted_us"] / 1e3,
                     "predicted_ms": stl["predicted_us"] / 1e3, "plan_ms": stl["plan_ms"], "norm_error": norml - 1.0}
                 model.free()
         del b32

@@ -114,8 +114,13 @@ def emulate(plan: HostPlan, psi: np.ndarray, precision: int = 64) -> np.ndarray:
             m = np.array(st.mat8[:2 * d * d]).view(np.complex128).reshape(d, d)
             qubits = [st.qubits[0]] if d == 2 else [st.qubits[0], st.qubits[1]]
             psi = _apply_matrix(psi, n, qubits, m, int(st.cmask))
-        elif st.kind == 7:      # involutive qubit permutation: bit q moves to sigma[q]
+        elif st.kind == 7:      # 1q gates on tile qubits, then the involutive permutation (bit q -> sigma[q])
             sigma = np.frombuffer(plan.payload, dtype=np.uint32, count=n, offset=st.dev_off)
+            gq = np.frombuffer(plan.payload, dtype=np.uint32, count=st.k, offset=st.dev_off2)
+            gm = np.frombuffer(plan.payload, dtype=np.float64, count=8 * st.k, offset=st.dev_off3)
+            for g in range(st.k):
+                m = gm[8 * g:8 * g + 8].view(np.complex128).reshape(2, 2)
+                psi = _apply_matrix(psi, n, [int(gq[g])], m, 0)
             dst = np.zeros(1 << n, dtype=np.int64)
             for q in range(n):
                 dst |= ((idx >> q) & 1) << int(sigma[q])

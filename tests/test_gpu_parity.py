@@ -154,6 +154,41 @@ def test_qft_gaussian_planned(hip, orc, n):
     assert max_abs(got, oracle_run(orc, c, psi)) < PARITY_TOL
 
 
+@pytest.mark.parametrize("seed", range(8))
+def test_trailing_swap_involution_sweep(hip, hip32, orc, seed):
+    """A program ending in disjoint SWAPs (at least two) runs them as one
+    in-place permutation sweep (launch kind 7), with the one-qubit gates
+    just before them on tile qubits applied in the same sweep; the result
+    equals the oracle's gate-by-gate run in both precisions."""
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(13, 21))
+    c = random_circuit(n, 30, rng, kq=False, controls=False)
+    perm = [int(q) for q in rng.permutation(n)]
+    pairs = int(rng.integers(2, n // 2 + 1))
+    low = [q for q in range(min(5, n))]
+    for q in low[: int(rng.integers(0, 4))]:
+        c.h(q)
+    if seed % 2:
+        c.ry(0.3 + seed, perm[0])
+    for j in range(pairs):
+        c.swap(perm[2 * j], perm[2 * j + 1])
+    psi = W.gaussian_state(n, seed)
+    want = oracle_run(orc, c, psi)
+    print(f"bitperm seed={seed} n={n} pairs={pairs}", flush=True)
+    for b, tol in ((hip, PARITY_TOL), (hip32, 2e-5)):
+        for jit in (False, True):
+            st = b.allocate(n)
+            st.load(psi)
+            plan = b.compile(c, jit=jit)
+            _, kinds = plan.execute_timed(st)
+            b.synchronize()
+            plan.free()
+            got = st.amplitudes()
+            st.free()
+            assert kinds[-1] == 7
+            assert max_abs(got, want) < tol
+
+
 @pytest.mark.parametrize("seed", range(20))
 def test_random_circuits_planned(hip, orc, seed):
     rng = np.random.default_rng(seed)

@@ -203,7 +203,7 @@ def test_trailing_swap_involution_is_one_permutation_step(oracle_lib, seed):
     """A program ending in disjoint SWAPs plans them as one in-place qubit
     permutation (step kind 7); anything else keeps the swap passes."""
     rng = np.random.default_rng(seed)
-    n = int(rng.integers(6, 13))
+    n = int(rng.integers(13, 16))
     c = W.brickwall(n, 3, seed=seed)
     perm = rng.permutation(n)
     pairs = [(int(perm[2 * i]), int(perm[2 * i + 1])) for i in range(int(rng.integers(2, n // 2 + 1)))]
@@ -222,6 +222,25 @@ def test_trailing_swap_involution_is_one_permutation_step(oracle_lib, seed):
 
 
 def test_qft_final_reversal_is_one_permutation_step():
-    for n in (10, 30):
+    for n in (16, 30):
         kinds = [s.kind for s in HostPlan(W.qft(n)).steps]
         assert kinds[-1] == 7 and kinds.count(7) == 1
+
+
+def test_tile_gates_fold_into_the_permutation(oracle_lib):
+    """One-qubit gates on tile qubits just before a final SWAP involution run
+    inside the permutation sweep: the QFT's last Hadamard no longer costs a
+    pass of its own."""
+    hp = HostPlan(W.qft(30))
+    assert hp.steps[-1].kind == 7 and hp.steps[-1].k >= 1
+    assert len(hp.steps) == len(HostPlan(W.qft(30, swaps=False)).steps)
+    assert 7 not in [s.kind for s in HostPlan(W.qft(12)).steps]     # one chunk: swaps are register renames
+    n = 16
+    hp = HostPlan(W.qft(n))
+    assert hp.steps[-1].kind == 7 and hp.steps[-1].k >= 1
+    psi = W.gaussian_state(n, 3)
+    assert max_abs(emulate(hp, psi), oracle_state(oracle_lib, W.qft(n), psi)) < 1e-12
+    c = W.brickwall(n, 2, seed=1).h(0).rx(0.3, n - 1).u3(0.1, 0.2, 0.3, 1).swap(0, n - 1).swap(1, n - 2)
+    hp = HostPlan(c)
+    assert hp.steps[-1].kind == 7 and hp.steps[-1].k == 3
+    assert max_abs(emulate(hp, psi), oracle_state(oracle_lib, c, psi)) < 1e-12
