@@ -89,7 +89,8 @@ class BatchStats(C.Structure):
     _fields_ = [("plan_ms", f64), ("sweep_bytes", f64), ("pool_bytes", u64), ("n_instances", u32),
                 ("n_passes", u32), ("n_items", u32), ("n_rounds", u32), ("grid", u32), ("threads", u32),
                 ("lds_bytes", u32), ("mode", u32), ("local_qubits", u32), ("chunks_per_item", u32),
-                ("groups_per_cu", u32), ("solo_instances", u32)]
+                ("groups_per_cu", u32), ("solo_instances", u32), ("prepared_instances", u32),
+                ("prepared_qubits", u32)]
 
 
 class BatchTune(C.Structure):
@@ -104,7 +105,8 @@ class BatchView(C.Structure):
                 ("ops", P), ("n_ops", u32), ("op_bytes", u32),
                 ("terms", P), ("n_terms", u32), ("term_bytes", u32),
                 ("offsets", PU64), ("n_instances", u32), ("n_rounds", u32),
-                ("round_begin", PU32), ("threads", u32), ("lds_bytes", u32), ("total_amps", u64)]
+                ("round_begin", PU32), ("threads", u32), ("lds_bytes", u32), ("total_amps", u64),
+                ("plus_masks", PU64)]
 
 
 CHANNEL_PAULI, CHANNEL_KRAUS = 0, 1

@@ -69,6 +69,35 @@ std::vector<ProgInstr> qaoa_program(uint32_t n, const uint32_t* edges, uint32_t 
   return prog;
 }
 
+uint64_t strip_leading_hadamards(std::vector<ProgInstr>& prog, uint32_t n) {
+  const uint64_t all = n >= 64 ? ~0ull : (1ull << n) - 1ull;
+  uint64_t touched = 0, plus = 0;
+  size_t w = 0, i = 0;
+  for (; i < prog.size() && touched != all; i++) {
+    ProgInstr& pi = prog[i];
+    if (pi.type == ProgInstr::GATE) {
+      const DenseGate& g = pi.gate;
+      if (g.kind == ALEPH_HIP_GATE_H && g.controls.empty() && g.qubits.size() == 1 &&
+          !((touched >> g.qubits[0]) & 1ull)) {
+        plus |= 1ull << g.qubits[0];
+        touched |= 1ull << g.qubits[0];
+        continue;
+      }
+      for (uint32_t q : g.qubits) touched |= 1ull << q;
+      for (uint32_t q : g.controls) touched |= 1ull << q;
+    } else if (pi.type == ProgInstr::DIAG_PHASE) {
+      for (const auto& c : pi.conds)
+        for (uint64_t mk : c) touched |= mk;
+    }
+    if (w != i) prog[w] = std::move(pi);
+    w++;
+  }
+  for (; i < prog.size(); i++, w++)
+    if (w != i) prog[w] = std::move(prog[i]);
+  prog.resize(w);
+  return plus;
+}
+
 CutDesc cut_desc(uint32_t n, const uint32_t* edges, uint32_t n_edges) {
   CutDesc d{};
   d.n = n;
@@ -99,7 +128,7 @@ CutDesc cut_desc(uint32_t n, const uint32_t* edges, uint32_t n_edges) {
 Batch::~Batch() {
   drop_graph();
   for (void* p : {(void*)pool, (void*)d_passes, (void*)d_stages, (void*)d_ops, (void*)d_terms, (void*)d_items,
-                  (void*)d_offs, (void*)d_done, (void*)d_ctl})
+                  (void*)d_offs, (void*)d_prep, (void*)d_done, (void*)d_ctl})
     if (p) (void)hipFree(p);
 }
 
@@ -119,6 +148,7 @@ void Batch::plan() {
     throw make_failure(ALEPH_HIP_ERR_INVALID_ARGUMENT, "batch", "local_qubits must be in [5, 13]");
   }
   std::vector<std::unique_ptr<Plan>> plans(N);
+  plus_mask.assign(N, 0);
   std::mutex mu;
   std::exception_ptr first_error;
   std::atomic<uint32_t> next{0};
@@ -139,15 +169,16 @@ void Batch::plan() {
           po.jit = true;
           po.generic_phase = true;    // instances of one size share their pass kernels
         }
-        std::unique_ptr<Plan> p;
+        std::vector<ProgInstr> prog;
         if (qaoa.empty()) {
-          p = build_plan(progs[i], nq[i], precision, po);
+          prog = progs[i];
         } else {
           const QaoaSpec& q = qaoa[i];
-          p = build_plan(qaoa_program(nq[i], q.edges.data(), (uint32_t)(q.edges.size() / 2),
-                                      (uint32_t)q.gammas.size(), q.gammas.data(), q.betas.data()),
-                         nq[i], precision, po);
+          prog = qaoa_program(nq[i], q.edges.data(), (uint32_t)(q.edges.size() / 2), (uint32_t)q.gammas.size(),
+                              q.gammas.data(), q.betas.data());
         }
+        plus_mask[i] = strip_leading_hadamards(prog, nq[i]);
+        std::unique_ptr<Plan> p = build_plan(prog, nq[i], precision, po);
         for (const Step& s : p->steps)
           if (!own && s.kind != Step::FUSED)
             throw make_failure(ALEPH_HIP_ERR_UNSUPPORTED_INSTRUCTION, "batch",
@@ -293,6 +324,16 @@ void Batch::upload(hipStream_t s, int device) {
     ALEPH_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&d_ctl), sizeof(BatchCtl)));
   }
   if (N) ALEPH_HIP_CHECK(hipMemcpyAsync(d_offs, offs.data(), N * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  prep.resize(N);
+  for (uint32_t i = 0; i < N; i++) {
+    prep[i] = BatchPrep{};
+    prep[i].state_off = offs[i];
+    prep[i].plus = plus_mask[i];
+    prep[i].amp = std::ldexp(1.0, -(int)(__builtin_popcountll(plus_mask[i]) / 2));
+    if (__builtin_popcountll(plus_mask[i]) & 1) prep[i].amp *= std::sqrt(0.5);
+    prep[i].n = nq[i];
+  }
+  ensure_upload(prep, &d_prep, &cap_prep, s, &realloc);
   if (realloc) drop_graph();
   for (auto& sp : solo) {
     if (!sp->uploaded) sp->upload(s);
@@ -314,7 +355,7 @@ namespace {
 template <typename T>
 void record_batch(Batch& b, hipStream_t s) {
   const uint32_t N = b.size();
-  launch_batch_init<T>(s, b.pool, b.total_amps, b.d_offs, N, b.d_done, b.d_ctl);
+  launch_batch_init<T>(s, b.pool, b.d_prep, N, b.d_done, b.d_ctl);
   if (b.cfg.mode == BATCH_PERSISTENT) {
     launch_batch_persistent<T>(s, b.launch, b.pool, b.d_items, (uint32_t)b.items.size(), b.grid, b.d_passes,
                                b.d_stages, b.d_ops, b.d_terms, b.d_done, b.d_ctl,
@@ -339,7 +380,7 @@ Batch::GraphKey Batch::current_key() const {
   k.lds_bytes = launch.lds_bytes;
   k.nontemporal = launch.nontemporal;
   k.round_begin = round_begin;
-  k.ptrs = {pool, d_passes, d_stages, d_ops, d_terms, d_items, d_offs, d_done, d_ctl};
+  k.ptrs = {pool, d_passes, d_stages, d_ops, d_terms, d_items, d_offs, d_prep, d_done, d_ctl};
   return k;
 }
 

@@ -59,6 +59,10 @@ struct Batch {
   std::vector<BatchItem> items;
   std::vector<uint32_t> round_begin;   // items of pass round r: [round_begin[r], round_begin[r+1])
   std::vector<uint64_t> offs;          // first amplitude of each instance
+  // Qubits whose first instruction was an uncontrolled H, removed from the
+  // plan: the reset prepares them in |+> (every amplitude with no bit set
+  // outside the mask equals 2^(-|mask|/2), the rest 0).
+  std::vector<uint64_t> plus_mask;
   std::vector<uint32_t> inst_passes;   // passes per instance
   // Instances run as their own plans (generated kernels), after the packed ones.
   std::vector<std::unique_ptr<Plan>> solo;
@@ -80,9 +84,11 @@ struct Batch {
   PhaseTerm* d_terms = nullptr;
   BatchItem* d_items = nullptr;
   uint64_t* d_offs = nullptr;
+  BatchPrep* d_prep = nullptr;
   uint32_t* d_done = nullptr;
   BatchCtl* d_ctl = nullptr;
-  size_t cap_passes = 0, cap_stages = 0, cap_ops = 0, cap_terms = 0, cap_items = 0;
+  std::vector<BatchPrep> prep;
+  size_t cap_passes = 0, cap_stages = 0, cap_ops = 0, cap_terms = 0, cap_items = 0, cap_prep = 0;
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_src = nullptr;
   // What the captured graph baked in: launch geometry, per-round item
@@ -116,6 +122,12 @@ struct Batch {
   bool failed(hipStream_t s);
   void drop_graph();
 };
+
+// Remove every uncontrolled H that is the first instruction on its qubit
+// (barriers aside) and return the mask of those qubits.  On |0..0> these
+// gates only prepare |+> on the masked qubits, which the batch reset writes
+// directly, saving the passes that applied them.
+uint64_t strip_leading_hadamards(std::vector<ProgInstr>& prog, uint32_t n);
 
 // Build the program of one QAOA MaxCut instance: H on every qubit, then per
 // layer CNOT(u,v) Rz(gamma)(v) CNOT(u,v) on every edge and Rx(beta) on

@@ -1548,6 +1548,10 @@ aleph_hip_status aleph_hip_batch_stats_get(const aleph_hip_batch* h, aleph_hip_b
   s.chunks_per_item = b.cfg.chunks_per_item;
   s.groups_per_cu = b.cfg.groups_per_cu;
   s.solo_instances = (uint32_t)b.solo.size();
+  for (uint64_t mk : b.plus_mask) {
+    s.prepared_instances += mk ? 1u : 0u;
+    s.prepared_qubits += (uint32_t)__builtin_popcountll(mk);
+  }
   *out = s;
   return ALEPH_HIP_OK;
 }
@@ -1661,6 +1665,7 @@ aleph_hip_status aleph_hip_batch_view_get(const aleph_hip_batch* h, aleph_hip_ba
   v.n_instances = b.size();
   v.n_rounds = b.round_begin.empty() ? 0 : (uint32_t)b.round_begin.size() - 1;
   v.round_begin = b.round_begin.data();
+  v.plus_masks = b.plus_mask.data();
   v.threads = b.launch.threads;
   v.lds_bytes = (uint32_t)b.launch.lds_bytes;
   v.total_amps = b.total_amps;

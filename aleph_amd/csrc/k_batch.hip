@@ -103,24 +103,23 @@ __global__ void __launch_bounds__(BT, 2)
   }
 }
 
+// Every instance's initial state in one write: blockIdx.y (grid-stride)
+// picks the instance, the x dimension strides over its amplitudes.
 template <typename T>
-__global__ void __launch_bounds__(256) k_batch_zero(C2<T>* __restrict__ pool, uint64_t n_amps) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  C2<T> z;
-  z.x = 0;
-  z.y = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_amps; i += stride) pool[i] = z;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_batch_basis(C2<T>* __restrict__ pool, const uint64_t* __restrict__ offs,
-                                                     uint32_t n_inst) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_inst) return;
-  C2<T> one;
-  one.x = 1;
-  one.y = 0;
-  pool[offs[i]] = one;
+__global__ void __launch_bounds__(256) k_batch_prep(C2<T>* __restrict__ pool, const BatchPrep* __restrict__ prep,
+                                                    uint32_t n_inst) {
+  for (uint32_t i = blockIdx.y; i < n_inst; i += gridDim.y) {
+    const BatchPrep d = prep[i];
+    const uint64_t size = 1ull << d.n;
+    C2<T>* st = pool + d.state_off;
+    for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < size;
+         x += (uint64_t)gridDim.x * blockDim.x) {
+      C2<T> v;
+      v.x = (x & ~d.plus) == 0 ? (T)d.amp : (T)0;
+      v.y = 0;
+      st[x] = v;
+    }
+  }
 }
 
 __global__ void __launch_bounds__(256) k_batch_reset(uint32_t* __restrict__ done, uint32_t n_inst,
@@ -303,16 +302,13 @@ int batch_persistent_occupancy(const BatchLaunch& cfg) {
 }
 
 template <typename T>
-void launch_batch_init(hipStream_t s, void* pool, uint64_t n_amps, const uint64_t* offs, uint32_t n_inst,
-                       uint32_t* done, BatchCtl* ctl) {
-  const uint64_t want = (n_amps + 255) / 256;
-  const uint32_t blocks = (uint32_t)(want < 65536 ? (want ? want : 1) : 65536);
-  hipLaunchKernelGGL((k_batch_zero<T>), dim3(blocks), dim3(256), 0, s, reinterpret_cast<C2<T>*>(pool), n_amps);
-  ALEPH_HIP_CHECK(hipGetLastError());
-  const uint32_t ib = (n_inst + 255) / 256;
-  hipLaunchKernelGGL((k_batch_basis<T>), dim3(ib ? ib : 1), dim3(256), 0, s, reinterpret_cast<C2<T>*>(pool), offs,
-                     n_inst);
-  ALEPH_HIP_CHECK(hipGetLastError());
+void launch_batch_init(hipStream_t s, void* pool, const BatchPrep* prep, uint32_t n_inst, uint32_t* done,
+                       BatchCtl* ctl) {
+  if (n_inst) {
+    hipLaunchKernelGGL((k_batch_prep<T>), dim3(64, std::min<uint32_t>(n_inst, 65535)), dim3(256), 0, s,
+                       reinterpret_cast<C2<T>*>(pool), prep, n_inst);
+    ALEPH_HIP_CHECK(hipGetLastError());
+  }
   launch_batch_reset(s, done, n_inst, ctl);
 }
 
@@ -341,8 +337,7 @@ void launch_batch_cut(hipStream_t s, const void* pool, const CutDesc* descs, uin
                                            const DevOp*, const PhaseTerm*, uint32_t*, BatchCtl*, uint32_t);  \
   template int batch_persistent_occupancy<T>(const BatchLaunch&);                                            \
   template void batch_configure<T>(const BatchLaunch&);                                                      \
-  template void launch_batch_init<T>(hipStream_t, void*, uint64_t, const uint64_t*, uint32_t, uint32_t*,     \
-                                     BatchCtl*);                                                             \
+  template void launch_batch_init<T>(hipStream_t, void*, const BatchPrep*, uint32_t, uint32_t*, BatchCtl*);  \
   template void launch_batch_cut<T>(hipStream_t, const void*, const CutDesc*, uint32_t, uint32_t, double*,   \
                                     double*);
 ALEPH_BATCH_INST(double)

@@ -409,11 +409,23 @@ __global__ void __launch_bounds__(kThreads) k_scale_by_norm(C2<T>* __restrict__ 
   const double t = norm[0] + norm[1];
   if (norm[2] != 0.0 || !(t > 0.0) || !isfinite(t)) return;
   const double f = 1.0 / sqrt(t);
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
-    C2<T> a = s[i];
-    a.x = (T)((double)a.x * f);
-    a.y = (T)((double)a.y * f);
-    s[i] = a;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < len; i0 += kUnroll * stride) {
+    C2<T> a[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const uint64_t i = i0 + u * stride;
+      if (i < len) a[u] = s[i];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const uint64_t i = i0 + u * stride;
+      if (i < len) {
+        a[u].x = (T)((double)a[u].x * f);
+        a[u].y = (T)((double)a[u].y * f);
+        s[i] = a[u];
+      }
+    }
   }
 }
 
@@ -556,7 +568,8 @@ template <typename T>
 void launch_normalize(hipStream_t st, void* state, uint32_t n, double* partials, double* res) {
   launch_reduce_branch<T>(st, state, n, -1, partials, res);
   const uint64_t len = 1ull << n;
-  hipLaunchKernelGGL(k_scale_by_norm<T>, dim3(grid_for(len)), dim3(kThreads), 0, st, (C2<T>*)state, len,
+  hipLaunchKernelGGL(k_scale_by_norm<T>, dim3(grid_for((len + kUnroll - 1) / kUnroll)), dim3(kThreads), 0, st,
+                     (C2<T>*)state, len,
                      (const double*)res);
   ALEPH_HIP_CHECK(hipGetLastError());
 }

@@ -85,10 +85,10 @@ void launch_batch_persistent(hipStream_t s, const BatchLaunch& cfg, void* pool, 
 template <typename T> int batch_persistent_occupancy(const BatchLaunch& cfg);
 // Allow the LDS size of `cfg` on both executors (call before graph capture).
 template <typename T> void batch_configure(const BatchLaunch& cfg);
-// Zero the pool, set amplitude 0 of every instance to 1, reset the queue.
+// Write every instance's initial state (BatchPrep) and reset the queue.
 template <typename T>
-void launch_batch_init(hipStream_t s, void* pool, uint64_t n_amps, const uint64_t* offs, uint32_t n_inst,
-                       uint32_t* done, BatchCtl* ctl);
+void launch_batch_init(hipStream_t s, void* pool, const BatchPrep* prep, uint32_t n_inst, uint32_t* done,
+                       BatchCtl* ctl);
 void launch_batch_reset(hipStream_t s, uint32_t* done, uint32_t n_inst, BatchCtl* ctl);
 // MaxCut expectation per instance: out[i] = sum_x |psi_x|^2 cut_i(x); one
 // partial per 4096-amplitude block (descs[i].block_begin numbers them).

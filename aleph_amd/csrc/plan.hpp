@@ -111,6 +111,16 @@ struct alignas(16) BatchItem {
   uint32_t pad;
 };
 
+// Initial state of one instance: amplitude x of the 2^n at state_off is
+// amp when x has no bit outside plus (|0..0> for plus = 0), else 0.
+struct alignas(16) BatchPrep {
+  uint64_t state_off;
+  uint64_t plus;
+  double amp;
+  uint32_t n;
+  uint32_t pad;
+};
+
 // Queue head and failure flag of the persistent executor.
 struct alignas(16) BatchCtl {
   uint32_t head;

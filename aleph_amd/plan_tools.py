@@ -183,6 +183,7 @@ class HostBatch:
             self.offsets = [int(v.offsets[i]) for i in range(v.n_instances)]
             self.round_begin = [int(v.round_begin[i]) for i in range(v.n_rounds + 1)]
             self.threads, self.lds_bytes, self.total_amps = v.threads, v.lds_bytes, v.total_amps
+            self.plus_masks = [int(v.plus_masks[i]) for i in range(v.n_instances)]
         finally:
             lib.aleph_hip_batch_free(h)
         self.num_qubits = nq.tolist()

@@ -187,6 +187,7 @@ pub struct aleph_hip_batch_stats {
     pub n_instances: u32, pub n_passes: u32, pub n_items: u32, pub n_rounds: u32,
     pub grid: u32, pub threads: u32, pub lds_bytes: u32, pub mode: u32,
     pub local_qubits: u32, pub chunks_per_item: u32, pub groups_per_cu: u32, pub solo_instances: u32,
+    pub prepared_instances: u32, pub prepared_qubits: u32,
 }
 
 #[repr(C)]
@@ -210,6 +211,7 @@ pub struct aleph_hip_batch_view {
     pub round_begin: *const u32,
     pub threads: u32, pub lds_bytes: u32,
     pub total_amps: u64,
+    pub plus_masks: *const u64,
 }
 
 /// One noise channel attached after a program record (noise trajectories).
@@ -322,6 +324,7 @@ mod tests {
         assert_eq!(std::mem::size_of::<aleph_hip_gate>(), 64);
         assert_eq!(std::mem::size_of::<aleph_hip_plan_opts>(), 32);
         assert_eq!(std::mem::size_of::<aleph_hip_batch_opts>(), 32);
+        assert_eq!(std::mem::size_of::<aleph_hip_batch_stats>(), 80);
         assert_eq!(std::mem::size_of::<aleph_hip_channel>(), 48);
     }
 }

@@ -371,6 +371,8 @@ typedef struct aleph_hip_batch_stats {
   uint32_t n_instances, n_passes, n_items, n_rounds;
   uint32_t grid, threads, lds_bytes, mode;
   uint32_t local_qubits, chunks_per_item, groups_per_cu, solo_instances;
+  uint32_t prepared_instances; /* instances whose leading H gates became the reset's |+> preparation */
+  uint32_t prepared_qubits;    /* those H gates, summed over the instances */
 } aleph_hip_batch_stats;
 typedef struct aleph_hip_batch_tune {
   uint32_t mode, local_qubits, chunks_per_item, groups_per_cu;
@@ -426,6 +428,7 @@ typedef struct aleph_hip_batch_view {
   const uint32_t* round_begin;                              /* n_rounds + 1 item indices */
   uint32_t threads, lds_bytes;
   uint64_t total_amps;
+  const uint64_t* plus_masks;                               /* per instance: qubits prepared in |+> */
 } aleph_hip_batch_view;
 aleph_hip_status aleph_hip_batch_view_get(const aleph_hip_batch* b, aleph_hip_batch_view* out);
 

@@ -43,7 +43,7 @@ def test_struct_layouts():
     assert C.sizeof(L.PlanOpts) == 8 * 4
     assert C.sizeof(L.Error) == 4 * 5 + 4 + 8 + 48 + 256
     assert C.sizeof(L.BatchOpts) == 8 * 4
-    assert C.sizeof(L.BatchStats) == 8 * 3 + 12 * 4
+    assert C.sizeof(L.BatchStats) == 8 * 3 + 14 * 4
     assert C.sizeof(L.BatchTune) == 4 * 4 + 8
     assert C.sizeof(L.Channel) == 6 * 4 + 3 * 8
 

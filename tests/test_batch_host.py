@@ -14,6 +14,26 @@ from tests.helpers import max_abs
 from tests.plan_emulator import emulate
 
 
+def prepared_state(n, plus):
+    """The batch reset's state: 2^(-|plus|/2) on every index with no bit outside `plus`."""
+    x = np.arange(1 << n, dtype=np.uint64)
+    psi = np.where((x & np.uint64(~plus & ((1 << n) - 1))) == 0, 1.0, 0.0).astype(np.complex128)
+    return psi / np.sqrt(float(1 << bin(plus).count("1")))
+
+
+def test_leading_hadamards_become_the_reset():
+    """The QAOA H layer is not planned as passes: instance plans lose the
+    passes that applied it (n=16, 12 local qubits: 11 -> 9 passes at p=4)
+    and record every qubit in the instance's |+> mask."""
+    graphs, gammas, betas = W.qaoa_batch(4, seed=3, n_min=16, n_max=16, p=4)
+    hb = HostBatch(graphs, gammas, betas, precision=32, local_qubits=12)
+    assert hb.plus_masks == [(1 << 16) - 1] * 4
+    from aleph_amd.plan_tools import HostPlan
+    for i, (n, edges) in enumerate(graphs):
+        full = HostPlan(W.qaoa_maxcut(n, edges, gammas[i], betas[i]), precision=32, local_qubits=12)
+        assert len(hb.instance_passes[i]) < full.stats["n_passes"]
+
+
 @pytest.mark.parametrize("local_qubits,cpi,n_max", [(0, 0, 22), (9, 1, 16), (11, 4, 22), (13, 64, 24)])
 def test_batch_items_cover_and_order(local_qubits, cpi, n_max):
     graphs, gammas, betas = W.qaoa_batch(40, seed=17, n_min=6, n_max=n_max, p=4)
@@ -56,8 +76,8 @@ def test_batch_instances_emulate_to_oracle(oracle_lib):
         hb = HostBatch(graphs, gammas, betas, precision=prec, local_qubits=7)
         for i, (n, edges) in enumerate(graphs):
             c = W.qaoa_maxcut(n, edges, gammas[i], betas[i])
-            psi0 = np.zeros(1 << n, dtype=np.complex128)
-            psi0[0] = 1.0
+            assert hb.plus_masks[i] == (1 << n) - 1      # the H layer became the reset's |+>^n
+            psi0 = prepared_state(n, hb.plus_masks[i])
             o = oracle_lib.OracleSv()
             st = o.allocate(n)
             o.run(c, st)

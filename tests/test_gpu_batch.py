@@ -84,6 +84,62 @@ def test_generic_circuit_batch(hip, orc):
     b.free()
 
 
+def leading_hadamard_mask(c):
+    """Qubits whose first instruction is an uncontrolled H."""
+    from aleph_amd.circuit import DiagonalPhase, GateInstance
+    touched = plus = 0
+    for inst in c.instructions:
+        if isinstance(inst, GateInstance):
+            if inst.gate.name == "H" and not inst.controls and not (touched >> inst.qubits[0]) & 1:
+                plus |= 1 << inst.qubits[0]
+            for q in inst.qubits + inst.controls:
+                touched |= 1 << q
+        elif isinstance(inst, DiagonalPhase):
+            for t in inst.terms:
+                for m in t.conds:
+                    touched |= m
+    return plus
+
+
+def test_leading_hadamards_prepared_by_the_reset(hip, orc):
+    """An uncontrolled H that is the first instruction on its qubit is left
+    out of the plan and the reset writes |+> there instead.  Kept as gates:
+    an H after another gate on its qubit, a controlled H, and H on a qubit
+    a diagonal phase touched first.  Every instance equals the oracle on
+    two runs (the second a graph replay), packed and solo alike."""
+    from aleph_amd.circuit import Circuit, DiagonalPhase, Gate, GateInstance, PhaseTerm
+    rng = np.random.default_rng(8)
+    circuits, expect = [], []
+    for k in range(8):
+        n = 6 + k
+        c = Circuit(n)
+        if k % 2:
+            c.cx(0, 1)                                   # 0 and 1 touched before their H
+        if k % 3 == 0:
+            c.add(GateInstance.controlled(Gate.H(), [2], [3]))
+        if k == 4:
+            c.add(DiagonalPhase(n, [PhaseTerm((1 << 5,), 0.7)]))
+        for q in range(n):
+            if rng.random() < 0.7:
+                c.h(q)
+        body = random_circuit(n, 30, rng, kq=False)
+        for inst in body.instructions:
+            c.add(inst)
+        expect.append(leading_hadamard_mask(c))
+        circuits.append(c)
+    assert all(expect) and any(m != (1 << c.num_qubits) - 1 for m, c in zip(expect, circuits))
+    for solo in (0, 10):
+        b = HipBatch.circuits(hip, circuits, precision=64, local_qubits=6, solo_qubits=solo)
+        st = b.stats()
+        assert st["prepared_instances"] == sum(1 for m in expect if m)
+        assert st["prepared_qubits"] == sum(bin(m).count("1") for m in expect)
+        for rep in range(2):
+            b.run()
+            for i, c in enumerate(circuits):
+                assert max_abs(b.state(i), oracle_amps(orc, c)) < PARITY_TOL, (solo, rep, i)
+        b.free()
+
+
 def test_graph_replay_with_angle_updates(hip, orc):
     """The captured graph is replayed after every angle update, with eager
     device work and synchronisations in between; each replay is checked."""
