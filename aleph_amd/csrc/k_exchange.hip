@@ -53,13 +53,24 @@ __global__ void __launch_bounds__(256) k_exchange(T* const* ranks, uint32_t m, u
 // sends the low five tile bits to the upper ones, 32 consecutive t would
 // otherwise hit LDS addresses 32 elements apart, all in one bank group.
 constexpr uint32_t kPermMaxTileBits = 10;
+// Both tiles are loaded into registers at once, kPermPerThread elements a
+// thread (eight: at n=30 a sweep takes 8.0 ms in FP64 against 8.2 ms at
+// four, 12 ms at two and 10 ms at sixteen, where registers limit the waves);
+// they then pass through one LDS tile in turn, so a workgroup holds a single
+// tile of LDS.
+constexpr uint32_t kPermPerThread = 8;
+// Every index the kernel needs is a bitwise or of precomputed parts: the
+// tile pair's base addresses are ors of per-block-index-bit contributions,
+// and element t = threadIdx.x + i * blockDim.x (blockDim a power of two)
+// splits into the thread's bits and the wave-uniform bits of i.
 struct PermMap {
-  uint32_t nt, nr;
+  uint32_t nt, nb;                      // tile bits; block-index bits
   uint32_t t_pos[kPermMaxTileBits];     // qubit of tile bit j (j < k: qubit j)
   uint32_t t_img[kPermMaxTileBits];     // tile bit that tile bit j moves to
-  uint32_t r_pos[40];
-  uint32_t r_img[40];
-  uint32_t r_from[40];                  // block-index bit that supplies bit j of the tile number
+  uint64_t blk_a[40];                   // block-index bit b set: or into tile r's base address
+  uint64_t blk_b[40];                   //   ... and into the partner tile sigma_R(r)'s
+  uint64_t off_i[kPermPerThread];       // address bits of element i * blockDim.x
+  uint32_t src_i[kPermPerThread];       // permuted tile index of element i * blockDim.x
   uint32_t ng;                          // one-qubit gates applied in LDS before permuting
   uint32_t g_bit[kPermMaxGates];        // tile bit each gate acts on
   double g_m[kPermMaxGates][8];
@@ -98,24 +109,24 @@ __device__ void tile_gates(T* v, const PermMap& pm, uint32_t tile) {
   }
 }
 
-// Both tiles are loaded into registers at once, kPermPerThread elements a
-// thread (eight: at n=30 a sweep takes 8.0 ms in FP64 against 8.2 ms at
-// four, 12 ms at two and 10 ms at sixteen, where registers limit the waves);
-// they then pass through one LDS tile in turn, so a workgroup holds a single
-// tile of LDS.
-constexpr uint32_t kPermPerThread = 8;
-
 template <typename T>
 __global__ void __launch_bounds__(128) k_bitperm(T* __restrict__ s, PermMap pm) {
   extern __shared__ unsigned char lds_raw[];
   T* lds = reinterpret_cast<T*>(lds_raw);
   const uint32_t tile = 1u << pm.nt;
-  uint32_t r = 0, r2 = 0;
-  for (uint32_t j = 0; j < pm.nr; j++) r |= ((blockIdx.x >> pm.r_from[j]) & 1u) << j;
-  for (uint32_t j = 0; j < pm.nr; j++) r2 |= ((r >> j) & 1u) << pm.r_img[j];
-  if (r2 < r) return;
-  const bool pair = r2 != r;
-  const uint64_t base_a = scatter_bits(r, pm.r_pos, pm.nr), base_b = scatter_bits(r2, pm.r_pos, pm.nr);
+  uint64_t base_a = 0, base_b = 0;
+  for (uint32_t b = 0; b < pm.nb; b++) {
+    if ((blockIdx.x >> b) & 1u) {
+      base_a |= pm.blk_a[b];
+      base_b |= pm.blk_b[b];
+    }
+  }
+  // Addresses order tiles as their numbers do (the tile-number bits sit at
+  // increasing qubits), so the pair is handled once, from its lower tile.
+  if (base_b < base_a) return;
+  const bool pair = base_b != base_a;
+  const uint64_t off_t = scatter_bits(threadIdx.x, pm.t_pos, pm.nt);
+  const uint32_t src_t = permute_bits(threadIdx.x, pm.t_img, pm.nt);
   // Components in plain arrays: arrays of the vector type stay in scratch
   // memory for FP64 instead of registers.
   using E = decltype(T::x);
@@ -124,7 +135,7 @@ __global__ void __launch_bounds__(128) k_bitperm(T* __restrict__ s, PermMap pm) 
   for (uint32_t i = 0; i < kPermPerThread; i++) {
     const uint32_t t = threadIdx.x + i * blockDim.x;
     if (t < tile) {
-      const uint64_t off = scatter_bits(t, pm.t_pos, pm.nt);
+      const uint64_t off = off_t | pm.off_i[i];
       const T u = s[base_a | off];
       ax[i] = u.x;
       ay[i] = u.y;
@@ -150,7 +161,7 @@ __global__ void __launch_bounds__(128) k_bitperm(T* __restrict__ s, PermMap pm) 
   for (uint32_t i = 0; i < kPermPerThread; i++) {
     const uint32_t t = threadIdx.x + i * blockDim.x;
     if (t < tile) {
-      const T u = lds[lds_slot(permute_bits(t, pm.t_img, pm.nt))];
+      const T u = lds[lds_slot(src_t | pm.src_i[i])];
       ax[i] = u.x;
       ay[i] = u.y;
     }
@@ -171,7 +182,7 @@ __global__ void __launch_bounds__(128) k_bitperm(T* __restrict__ s, PermMap pm) 
     for (uint32_t i = 0; i < kPermPerThread; i++) {
       const uint32_t t = threadIdx.x + i * blockDim.x;
       if (t < tile) {
-        const T u = lds[lds_slot(permute_bits(t, pm.t_img, pm.nt))];
+        const T u = lds[lds_slot(src_t | pm.src_i[i])];
         bx[i] = u.x;
         by[i] = u.y;
       }
@@ -181,7 +192,7 @@ __global__ void __launch_bounds__(128) k_bitperm(T* __restrict__ s, PermMap pm) 
   for (uint32_t i = 0; i < kPermPerThread; i++) {
     const uint32_t t = threadIdx.x + i * blockDim.x;
     if (t < tile) {
-      const uint64_t off = scatter_bits(t, pm.t_pos, pm.nt);
+      const uint64_t off = off_t | pm.off_i[i];
       T u, w;
       u.x = ax[i];
       u.y = ay[i];
@@ -210,27 +221,46 @@ void launch_bitperm(hipStream_t s, void* state, uint32_t n, const uint32_t* sigm
   for (uint32_t q = 0; q < k; q++) pm.t_pos[pm.nt++] = q;
   for (uint32_t q = k; q < n; q++)
     if ((tmask >> q) & 1ull) pm.t_pos[pm.nt++] = q;
+  uint32_t r_pos[64], nr = 0;
   for (uint32_t q = 0; q < n; q++)
-    if (!((tmask >> q) & 1ull)) pm.r_pos[pm.nr++] = q;
+    if (!((tmask >> q) & 1ull)) r_pos[nr++] = q;
   auto index_of = [](const uint32_t* pos, uint32_t count, uint32_t q) {
     for (uint32_t j = 0; j < count; j++)
       if (pos[j] == q) return j;
     return count;
   };
   for (uint32_t j = 0; j < pm.nt; j++) pm.t_img[j] = index_of(pm.t_pos, pm.nt, sigma[pm.t_pos[j]]);
-  for (uint32_t j = 0; j < pm.nr; j++) pm.r_img[j] = index_of(pm.r_pos, pm.nr, sigma[pm.r_pos[j]]);
   // Workgroups that run together take consecutive block indices, so the low
   // block-index bits go to the tile-number bits whose higher qubit, over the
   // bit and its image, is lowest: both tiles of neighbouring workgroups then
   // differ only in low address bits (the same pages), for the QFT's reversal
-  // the middle qubits first.  With r taken straight from the block index the
-  // partner tiles of neighbouring workgroups lie 2^(n-6) amplitudes apart.
-  uint32_t order[40];
-  for (uint32_t j = 0; j < pm.nr; j++) order[j] = j;
-  std::stable_sort(order, order + pm.nr, [&](uint32_t x, uint32_t y) {
-    return std::max(pm.r_pos[x], sigma[pm.r_pos[x]]) < std::max(pm.r_pos[y], sigma[pm.r_pos[y]]);
+  // the middle qubits first.  With the tile number taken straight from the
+  // block index, the partner tiles of neighbouring workgroups of the reversal
+  // lie 2^(n-6) amplitudes apart.
+  uint32_t order[64];
+  for (uint32_t j = 0; j < nr; j++) order[j] = j;
+  std::stable_sort(order, order + nr, [&](uint32_t x, uint32_t y) {
+    return std::max(r_pos[x], sigma[r_pos[x]]) < std::max(r_pos[y], sigma[r_pos[y]]);
   });
-  for (uint32_t b = 0; b < pm.nr; b++) pm.r_from[order[b]] = b;
+  if (nr > 40) throw Failure{ALEPH_HIP_ERR_INVALID_ARGUMENT, "bitperm: state too wide"};
+  pm.nb = nr;
+  for (uint32_t b = 0; b < nr; b++) {
+    const uint32_t q = r_pos[order[b]];
+    pm.blk_a[b] = 1ull << q;
+    pm.blk_b[b] = 1ull << sigma[q];
+  }
+  using V = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
+  const size_t tile = (size_t)1 << pm.nt;
+  const uint32_t threads = std::max<uint32_t>(64, (uint32_t)(tile / kPermPerThread));
+  for (uint32_t i = 0; i < kPermPerThread; i++) {
+    const uint32_t t = i * threads;
+    for (uint32_t j = 0; j < pm.nt; j++) {
+      if ((t >> j) & 1u) {
+        pm.off_i[i] |= 1ull << pm.t_pos[j];
+        pm.src_i[i] |= 1u << pm.t_img[j];
+      }
+    }
+  }
   if (n_gates > kPermMaxGates) throw Failure{ALEPH_HIP_ERR_INVALID_ARGUMENT, "bitperm: too many gates"};
   pm.ng = n_gates;
   for (uint32_t g = 0; g < n_gates; g++) {
@@ -238,11 +268,8 @@ void launch_bitperm(hipStream_t s, void* state, uint32_t n, const uint32_t* sigm
     if (pm.g_bit[g] >= pm.nt) throw Failure{ALEPH_HIP_ERR_INVALID_ARGUMENT, "bitperm: gate off the tile"};
     for (int e = 0; e < 8; e++) pm.g_m[g][e] = gate_mats[8 * g + e];
   }
-  using V = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
-  const size_t tile = (size_t)1 << pm.nt;
   const size_t lds = (tile + tile / 32 + 1) * sizeof(V);
-  const uint32_t threads = std::max<uint32_t>(64, (uint32_t)(tile / kPermPerThread));
-  hipLaunchKernelGGL(k_bitperm<V>, dim3(1u << pm.nr), dim3(threads), lds, s, (V*)state, pm);
+  hipLaunchKernelGGL(k_bitperm<V>, dim3(1u << pm.nb), dim3(threads), lds, s, (V*)state, pm);
   ALEPH_HIP_CHECK(hipGetLastError());
 }
 

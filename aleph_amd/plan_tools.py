@@ -11,7 +11,7 @@ from . import _lib as L
 
 u32, u64, f64 = C.c_uint32, C.c_uint64, C.c_double
 
-STEP_KINDS = ["fused", "kq", "diag", "poly", "1q", "2q", "swap"]
+STEP_KINDS = ["fused", "kq", "diag", "poly", "1q", "2q", "swap", "bitperm"]
 OP_NAMES = {1: "1q", 2: "2q", 3: "diag", 4: "sign", 5: "phase"}
 
 

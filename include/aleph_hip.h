@@ -218,7 +218,8 @@ aleph_hip_status aleph_hip_plan_execute(aleph_hip_ctx* ctx, aleph_hip_plan* plan
 /* Execute with a hipEvent pair around every launch; ms[i] receives the
  * device time of launch i (n_launches entries, see aleph_hip_plan_stats),
  * kinds[i] its kind (0 fused pass, 1 dense block, 2 diagonal, 3 phase
- * polynomial, 4 one-qubit, 5 two-qubit, 6 swap).  This is the timing
+ * polynomial, 4 one-qubit, 5 two-qubit, 6 swap, 7 trailing-SWAP permutation
+ * sweep).  This is the timing
  * recorder the plan cost models train on. */
 aleph_hip_status aleph_hip_plan_execute_timed(aleph_hip_ctx* ctx, aleph_hip_plan* plan, aleph_hip_state* st,
                                              double* ms, uint32_t* kinds);
