@@ -175,7 +175,7 @@ typedef struct aleph_hip_plan_opts {
   uint32_t planner;         /* ALEPH_HIP_PLANNER_* */
   uint32_t local_qubits;    /* qubits per LDS chunk (0 = 12) */
   uint32_t max_ops_per_pass;/* compute budget per pass, in 1q-gate equivalents (0 = default) */
-  uint32_t coalesce_bits;   /* low qubits kept in every chunk for coalescing (0 = default) */
+  uint32_t coalesce_bits;   /* low qubits kept in every chunk for coalescing (0 = default, 4) */
   uint32_t fuse_1q;         /* merge adjacent 1q gates on one qubit before planning */
   uint32_t reg_bits;        /* amplitudes per thread = 2^reg_bits, 4 or 5 (0 = default 4) */
   uint32_t flags;           /* ALEPH_HIP_PLAN_* bits */
