@@ -428,20 +428,35 @@ __device__ __forceinline__ double2 phase_class_uniform(const DevOp* __restrict__
 // Factor of one class for this thread: the workgroup's uniform factor `u`
 // times the unit factors of the class's COUNT thread-dependent terms, which
 // start REL terms after the op's first term.
+// The unit factors are selected independently and multiplied as a balanced
+// tree, so the dependent chain is log2(COUNT) + 1 products long instead of
+// COUNT: these passes run at two waves per SIMD and hide little latency.
 template <int REL, int COUNT>
 __device__ __forceinline__ double2 phase_factor(const DevOp* __restrict__ op, const PhaseTerm* __restrict__ terms,
                                                 uint64_t tgo, double2 u) {
-  const PhaseTerm* tp = terms + op->term_off + REL;
+  if constexpr (COUNT == 0) {
+    return u;
+  } else {
+    const PhaseTerm* tp = terms + op->term_off + REL;
+    double fc[COUNT], fs[COUNT];
 #pragma unroll
-  for (int j = 0; j < COUNT; j++) {
-    const uint64_t mo = tp[j].mask_out;
-    const bool on = (tgo & mo) == mo;
-    const double c = on ? tp[j].c : 1.0, sn = on ? tp[j].s : 0.0;
-    const double r = u.x * c - u.y * sn;
-    u.y = u.x * sn + u.y * c;
-    u.x = r;
+    for (int j = 0; j < COUNT; j++) {
+      const uint64_t mo = tp[j].mask_out;
+      const bool on = (tgo & mo) == mo;
+      fc[j] = on ? tp[j].c : 1.0;
+      fs[j] = on ? tp[j].s : 0.0;
+    }
+#pragma unroll
+    for (int w = 1; w < COUNT; w *= 2) {
+#pragma unroll
+      for (int j = 0; j + w < COUNT; j += 2 * w) {
+        const double r = fc[j] * fc[j + w] - fs[j] * fs[j + w];
+        fs[j] = fc[j] * fs[j + w] + fs[j] * fc[j + w];
+        fc[j] = r;
+      }
+    }
+    return make_double2(u.x * fc[0] - u.y * fs[0], u.x * fs[0] + u.y * fc[0]);
   }
-  return u;
 }
 
 template <typename T>
