@@ -67,7 +67,8 @@ class PlanStats(C.Structure):
     _fields_ = [("plan_ms", f64), ("input_gates", u32), ("n_passes", u32), ("n_stages", u32), ("n_ops", u32),
                 ("n_launches", u32), ("planner_used", u32), ("sweep_bytes", f64), ("chosen_local_qubits", u32),
                 ("chosen_max_ops", u32), ("chosen_flags", u32), ("n_candidates", u32), ("predicted_us", f64),
-                ("chosen_kq_width", u32), ("chosen_rule", u32), ("rule_predicted_us", f64)]
+                ("chosen_kq_width", u32), ("chosen_rule", u32), ("rule_predicted_us", f64),
+                ("cancelled_gates", u32), ("reserved", u32)]
 
 
 MODEL_INPUTS = 16

@@ -1213,6 +1213,85 @@ size_t tile_gates_before(const std::vector<ProgInstr>& prog, size_t cut, uint32_
   }
   return start;
 }
+
+// Whether gate b undoes gate a exactly: the same qubits in the same order,
+// the same controls, and b's matrix equal to a's conjugate transpose entry
+// for entry (Gate::inverse's closed forms give that for the self-inverse
+// kinds, S/Sdg, T/Tdg, Iswap/IswapDg and negated rotation angles).
+bool exact_inverse(const DenseGate& a, const DenseGate& b) {
+  if (a.qubits != b.qubits || a.dim != b.dim || a.m.size() != b.m.size()) return false;
+  std::vector<uint32_t> ca = a.controls, cb = b.controls;
+  std::sort(ca.begin(), ca.end());
+  std::sort(cb.begin(), cb.end());
+  if (ca != cb) return false;
+  const uint32_t d = a.dim;
+  for (uint32_t r = 0; r < d; r++)
+    for (uint32_t c = 0; c < d; c++)
+      if (b.m[(size_t)r * d + c] != std::conj(a.m[(size_t)c * d + r])) return false;
+  return true;
+}
+
+// CancelInversePairs (crates/aleph-ir/src/passes/cancel.rs:1-80) on the
+// decoded program: one forward pass with a stack of live instruction
+// indices per qubit.  A gate cancels with the live instruction on top of
+// every stack it uses when that instruction is a gate it undoes exactly;
+// both go.  Diagonal phases and barriers never cancel, so nothing cancels
+// across them (a barrier blocks every qubit: the decoded form keeps no
+// qubit list).
+std::vector<ProgInstr> cancel_inverse_pairs(const std::vector<ProgInstr>& prog, uint32_t n, uint32_t* cancelled) {
+  std::vector<std::vector<uint32_t>> live(n);
+  std::vector<char> dead(prog.size(), 0);
+  std::vector<uint32_t> sup;
+  uint32_t count = 0;
+  for (size_t i = 0; i < prog.size(); i++) {
+    const ProgInstr& pi = prog[i];
+    sup.clear();
+    if (pi.type == ProgInstr::GATE) {
+      sup = pi.gate.qubits;
+      sup.insert(sup.end(), pi.gate.controls.begin(), pi.gate.controls.end());
+    } else if (pi.type == ProgInstr::DIAG_PHASE) {
+      uint64_t mk = 0;
+      for (const auto& c : pi.conds)
+        for (uint64_t v : c) mk |= v;
+      for (uint32_t q = 0; q < n; q++)
+        if ((mk >> q) & 1ull) sup.push_back(q);
+    } else {
+      for (uint32_t q = 0; q < n; q++) sup.push_back(q);
+    }
+    if (pi.type == ProgInstr::GATE && !sup.empty()) {
+      int64_t cand = -1;
+      bool shared = true;
+      for (uint32_t q : sup) {
+        if (live[q].empty()) {
+          shared = false;
+          break;
+        }
+        const int64_t top = live[q].back();
+        if (cand < 0) cand = top;
+        else if (cand != top) {
+          shared = false;
+          break;
+        }
+      }
+      if (shared && cand >= 0 && prog[(size_t)cand].type == ProgInstr::GATE &&
+          exact_inverse(prog[(size_t)cand].gate, pi.gate)) {
+        dead[(size_t)cand] = 1;
+        dead[i] = 1;
+        for (uint32_t q : sup) live[q].pop_back();
+        count += 2;
+        continue;
+      }
+    }
+    for (uint32_t q : sup) live[q].push_back((uint32_t)i);
+  }
+  if (cancelled) *cancelled = count;
+  if (!count) return prog;
+  std::vector<ProgInstr> out;
+  out.reserve(prog.size() - count);
+  for (size_t i = 0; i < prog.size(); i++)
+    if (!dead[i]) out.push_back(prog[i]);
+  return out;
+}
 }  // namespace
 
 std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog_in, uint32_t n, uint32_t precision,
@@ -1233,7 +1312,9 @@ std::unique_ptr<Plan> build_plan(const std::vector<ProgInstr>& prog_in, uint32_t
     const uint32_t m_chunk = std::min<uint32_t>(opts.local_qubits ? opts.local_qubits : 12u, (uint32_t)kMaxLocalBits);
     const size_t cut = n > m_chunk ? trailing_swap_involution(prog_in, n, sigma) : prog_in.size();
     const size_t body = cut < prog_in.size() ? tile_gates_before(prog_in, cut, n, sigma) : cut;
-    const std::vector<ProgInstr> prog(prog_in.begin(), prog_in.begin() + (ptrdiff_t)body);
+    const std::vector<ProgInstr> prog =
+        cancel_inverse_pairs(std::vector<ProgInstr>(prog_in.begin(), prog_in.begin() + (ptrdiff_t)body), n,
+                             &p->stats.cancelled_gates);
     std::vector<ProgInstr> pre = opts.fuse_diag ? fuse_diagonal_runs(prog, n) : prog;
     if (opts.kq_width >= 3) pre = fuse_dense_blocks(pre, std::min<uint32_t>(opts.kq_width, kMaxFusedK));
     std::vector<LGate> lg = lower_program(pre, n);

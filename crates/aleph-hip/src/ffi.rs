@@ -153,6 +153,8 @@ pub struct aleph_hip_plan_stats {
     pub chosen_kq_width: u32,
     pub chosen_rule: u32,
     pub rule_predicted_us: f64,
+    pub cancelled_gates: u32,
+    pub reserved2: u32,
 }
 
 #[repr(C)]
@@ -325,6 +327,7 @@ mod tests {
         assert_eq!(std::mem::size_of::<aleph_hip_plan_opts>(), 32);
         assert_eq!(std::mem::size_of::<aleph_hip_batch_opts>(), 32);
         assert_eq!(std::mem::size_of::<aleph_hip_batch_stats>(), 80);
+        assert_eq!(std::mem::size_of::<aleph_hip_plan_stats>(), 88);
         assert_eq!(std::mem::size_of::<aleph_hip_channel>(), 48);
     }
 }

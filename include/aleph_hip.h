@@ -209,6 +209,8 @@ typedef struct aleph_hip_plan_stats {
   uint32_t chosen_kq_width;  /* LEARNED: dense-block width of the chosen candidate (0 = LDS passes) */
   uint32_t chosen_rule;      /* LEARNED: 1 when the rule planner's own options were kept */
   double rule_predicted_us;  /* LEARNED: predicted time of the rule planner's plan */
+  uint32_t cancelled_gates;  /* gates removed as adjacent exact-inverse pairs (CancelInversePairs) */
+  uint32_t reserved2;
 } aleph_hip_plan_stats;
 
 aleph_hip_status aleph_hip_plan_create(aleph_hip_ctx* ctx, uint32_t num_qubits, uint32_t precision,

@@ -45,6 +45,7 @@ def test_struct_layouts():
     assert C.sizeof(L.BatchOpts) == 8 * 4
     assert C.sizeof(L.BatchStats) == 8 * 3 + 14 * 4
     assert C.sizeof(L.BatchTune) == 4 * 4 + 8
+    assert C.sizeof(L.PlanStats) == 8 + 6 * 4 + 8 + 4 * 4 + 8 + 2 * 4 + 8 + 2 * 4
     assert C.sizeof(L.Channel) == 6 * 4 + 3 * 8
 
 

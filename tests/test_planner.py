@@ -247,3 +247,63 @@ def test_tile_gates_fold_into_the_permutation(oracle_lib):
     hp = HostPlan(c)
     assert hp.steps[-1].kind == 7 and hp.steps[-1].k == 3
     assert max_abs(emulate(hp, psi), oracle_state(oracle_lib, c, psi)) < 1e-12
+
+
+def _cancel_case(build, n=6):
+    c = Circuit(n)
+    build(c)
+    return c
+
+
+@pytest.mark.parametrize("name,build,cancelled", [
+    ("h_h", lambda c: c.h(0).h(0), 2),
+    ("s_sdg", lambda c: c.s(1).sdg(1), 2),
+    ("t_tdg_nested", lambda c: c.t(2).h(2).h(2).tdg(2), 4),
+    ("rz_neg", lambda c: c.rz(0.7, 3).rz(-0.7, 3), 2),
+    ("cnot_cnot", lambda c: c.cx(0, 1).cx(0, 1), 2),
+    ("cnot_reversed", lambda c: c.cx(0, 1).cx(1, 0), 0),
+    ("blocked_by_gate", lambda c: c.h(0).cx(0, 1).h(0), 0),
+    ("blocked_by_barrier", lambda c: c.h(0).barrier([0]).h(0), 0),
+    ("blocked_by_phase", lambda c: c.h(0).add(DiagonalPhase(6, [PhaseTerm((1,), 0.3)])).h(0), 0),
+    ("rx_not_inverse", lambda c: c.rx(0.7, 3).rx(0.7, 3), 0),
+    ("swap_swap_other_qubit_between", lambda c: c.swap(0, 5).h(2).swap(0, 5), 2),
+])
+def test_cancel_inverse_pairs(oracle_lib, name, build, cancelled):
+    """CancelInversePairs (crates/aleph-ir/src/passes/cancel.rs) in the GPU
+    planner: adjacent exact inverses on the same qubits cancel, nested pairs
+    cancel in one pass, and nothing cancels across a gate, a barrier or a
+    diagonal phase on those qubits.  The plan still equals the oracle's
+    gate-by-gate run."""
+    c = _cancel_case(build).u3(0.3, 0.2, 0.1, 4)       # keep the program non-empty after cancelling
+    hp = HostPlan(c)
+    assert hp.stats["cancelled_gates"] == cancelled, name
+    psi = W.gaussian_state(6, 5)
+    assert max_abs(emulate(hp, psi), oracle_state(oracle_lib, c, psi)) < 1e-12
+
+
+def test_cancel_inverse_pairs_random_mirrors(oracle_lib):
+    """U followed by its gate-by-gate inverse in reverse order cancels to
+    nothing for the closed-form kinds (each gate meets its inverse once the
+    inner pairs are gone)."""
+    rng = np.random.default_rng(3)
+    n = 7
+    c = Circuit(n)
+    inv = {"H": "H", "X": "X", "Y": "Y", "Z": "Z", "S": "Sdg", "Sdg": "S", "T": "Tdg", "Tdg": "T"}
+    seq = []
+    for _ in range(40):
+        if rng.random() < 0.6:
+            name = list(inv)[int(rng.integers(len(inv)))]
+            q = int(rng.integers(n))
+            seq.append(("1", name, q))
+        else:
+            a, b = (int(v) for v in rng.choice(n, 2, replace=False))
+            seq.append(("2", "Cnot" if rng.random() < 0.5 else "Cz", (a, b)))
+    for kind, name, q in seq:
+        c.gate(Gate(name), [q] if kind == "1" else list(q))
+    for kind, name, q in reversed(seq):
+        c.gate(Gate(inv[name] if kind == "1" else name), [q] if kind == "1" else list(q))
+    c.u3(0.3, 0.2, 0.1, 0)
+    hp = HostPlan(c)
+    assert hp.stats["cancelled_gates"] == 2 * len(seq)
+    psi = W.gaussian_state(n, 9)
+    assert max_abs(emulate(hp, psi), oracle_state(oracle_lib, c, psi)) < 1e-12
