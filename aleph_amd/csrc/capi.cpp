@@ -1050,13 +1050,22 @@ aleph_hip_status aleph_hip_expectation(aleph_hip_ctx* c, aleph_hip_state* st, do
       if (paulis[i] == 2) { x |= b; zy |= b; ny++; }
       if (paulis[i] == 3) zy |= b;
     }
+    // One term through the grouped kernel: for x != 0 it sums over the pairs
+    // {i, i ^ x}, so every amplitude is read once (a per-element read of
+    // psi[i] and psi[i ^ x] reads the state twice when x holds high qubits).
     double* partials = c->d_scratch;
     double* res = c->d_scratch + kResOffset;
-    if (st->precision == ALEPH_HIP_F64) launch_expect_pauli<double>(c->stream, st->d, st->n, x, zy, partials, res);
-    else launch_expect_pauli<float>(c->stream, st->d, st->n, x, zy, partials, res);
-    ALEPH_HIP_CHECK(hipMemcpyAsync(c->h_pinned, res, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    const uint64_t zys[1] = {zy};
+    const bool odd = x != 0 && (__builtin_popcountll(x & zy) & 1);
+    const uint8_t use_im[1] = {(uint8_t)odd};
+    if (st->precision == ALEPH_HIP_F64)
+      launch_expect_group<double>(c->stream, st->d, st->n, x, zys, use_im, 1u, partials, res);
+    else
+      launch_expect_group<float>(c->stream, st->d, st->n, x, zys, use_im, 1u, partials, res);
+    ALEPH_HIP_CHECK(hipMemcpyAsync(c->h_pinned, res, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     ALEPH_HIP_CHECK(hipStreamSynchronize(c->stream));
-    const double re = c->h_pinned[0], im = c->h_pinned[1];
+    const double acc = c->h_pinned[0];
+    const double re = x == 0 ? acc : (odd ? 0.0 : 2.0 * acc), im = odd ? 2.0 * acc : 0.0;
     double v;
     switch (ny & 3) {  // Re(i^ny * (re + i im))
       case 0: v = re; break;

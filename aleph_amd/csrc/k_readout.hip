@@ -133,25 +133,6 @@ __global__ void k_collapse(C2<T>* __restrict__ s, uint64_t len, uint64_t qmask, 
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kThreads) k_expect_pauli(const C2<T>* __restrict__ s, uint64_t len, uint64_t x,
-                                                           uint64_t zy, double* __restrict__ partials) {
-  KSum v[2] = {{0, 0}, {0, 0}};
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
-    const C2<T> a = s[i], b = s[i ^ x];
-    const double sg = (__popcll(i & zy) & 1) ? -1.0 : 1.0;
-    // conj(b) * a
-    kadd(v[0], sg * ((double)b.x * (double)a.x + (double)b.y * (double)a.y));
-    kadd(v[1], sg * ((double)b.x * (double)a.y - (double)b.y * (double)a.x));
-  }
-  block_reduce<2>(v);
-  if (threadIdx.x == 0) {
-    partials[3 * blockIdx.x + 0] = v[0].s + v[0].c;
-    partials[3 * blockIdx.x + 1] = v[1].s + v[1].c;
-    partials[3 * blockIdx.x + 2] = 0.0;
-  }
-}
-
 // All Pauli terms that share one X mask `x` in one read of the state.
 // With z = conj(psi[i ^ x]) psi[i] and s_t(i) = (-1)^{|i & zy_t|}, term t
 // needs R_t = sum_i s_t(i) Re z(i) and I_t = sum_i s_t(i) Im z(i).  For
@@ -520,18 +501,6 @@ void launch_collapse(hipStream_t st, void* state, uint32_t n, uint32_t qubit, in
 }
 
 template <typename T>
-void launch_expect_pauli(hipStream_t st, const void* state, uint32_t n, uint64_t xmask, uint64_t zymask,
-                         double* partials, double* out) {
-  const uint64_t len = 1ull << n;
-  uint32_t blocks = grid_for(len);
-  if (blocks > kReduceBlocks) blocks = kReduceBlocks;
-  hipLaunchKernelGGL(k_expect_pauli<T>, dim3(blocks), dim3(kThreads), 0, st, (const C2<T>*)state, len, xmask,
-                     zymask, partials);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, st, partials, blocks, 3u, out);
-  ALEPH_HIP_CHECK(hipGetLastError());
-}
-
-template <typename T>
 void launch_marginal(hipStream_t st, const void* state, uint32_t n, uint32_t k, const uint32_t* qubits, double* out) {
   const uint64_t len = 1ull << n;
   const uint64_t bins = 1ull << k;
@@ -597,7 +566,6 @@ template uint32_t launch_compare<float, float>(hipStream_t, const void*, const v
   template void launch_expect_group<T>(hipStream_t, const void*, uint32_t, uint64_t, const uint64_t*,            \
                                        const uint8_t*, uint32_t, double*, double*);                             \
   template void launch_collapse<T>(hipStream_t, void*, uint32_t, uint32_t, int32_t, double);                    \
-  template void launch_expect_pauli<T>(hipStream_t, const void*, uint32_t, uint64_t, uint64_t, double*, double*); \
   template void launch_marginal<T>(hipStream_t, const void*, uint32_t, uint32_t, const uint32_t*, double*);      \
   template void launch_sample<T>(hipStream_t, const void*, uint32_t, double*, double*, const double*, uint32_t,   \
                                  uint64_t*);

@@ -139,10 +139,7 @@ void launch_expect_group(hipStream_t s, const void* state, uint32_t n, uint64_t 
                          const uint8_t* use_im, uint32_t count, double* partials, double* out);
 template <typename T>
 void launch_collapse(hipStream_t s, void* state, uint32_t n, uint32_t qubit, int32_t outcome, double scale);
-// <psi| P |psi> pieces: out[0] = Re, out[1] = Im of sum conj(psi[i^x]) * (-1)^{|i & zy|} psi[i].
-template <typename T>
-void launch_expect_pauli(hipStream_t s, const void* state, uint32_t n, uint64_t xmask, uint64_t zymask,
-                         double* partials, double* out);
+
 // Marginal histogram over k qubits into `out` (2^k doubles, device, zeroed by the launcher).
 template <typename T>
 void launch_marginal(hipStream_t s, const void* state, uint32_t n, uint32_t k, const uint32_t* qubits,

@@ -278,6 +278,27 @@ def test_norm_measure_expectation_probabilities(hip, orc):
 
 
 @pytest.mark.parametrize("precision,tol", [(64, 1e-12), (32, 1e-5)])
+def test_expectation_single_strings_every_x_mask(orc, precision, tol):
+    """Backend::expectation_value (measure.rs:155-235) of single strings
+    whose X/Y parts sit on low, high and mixed qubits (the pair sums over
+    {i, i ^ x}), identity and Z-only strings (x = 0), with even and odd
+    numbers of Y: equal to the oracle."""
+    n = 16
+    hip = B.HipSvBackend(seed=4, precision=precision)
+    c = W.brickwall(n, 6, seed=11)
+    st = hip.allocate(n)
+    hip.run_program(st, c)
+    o = orc.OracleSv()
+    ost = o.run(c)
+    strings = [[], [(15, "X")], [(15, "Y")], [(0, "Y"), (15, "Y")], [(14, "Z"), (3, "X")],
+               [(15, "X"), (7, "Y"), (0, "Z")], [(2, "Z"), (9, "Z")], [(8, "Y"), (12, "Y"), (13, "Y")]]
+    for terms in strings:
+        ps = PauliString.new(1.3, terms)
+        assert abs(hip.expectation_value(st, ps) - o.expectation_value(ost, ps)) < tol, terms
+    st.free()
+
+
+@pytest.mark.parametrize("precision,tol", [(64, 1e-12), (32, 1e-5)])
 def test_expectation_pauli_sum(orc, precision, tol):
     """Backend::expectation_pauli_sum (crates/aleph-backend/src/lib.rs:422-432):
     one call over many Pauli strings equals the sum of the oracle's
