@@ -254,10 +254,11 @@ def _mirror_fidelity_lower_bound(b, st, n):
 
 def test_mirror_headline_plan_n30_fp64():
     n = 30
-    c = mirror(W.brickwall(n, 20, seed=2024))
+    c = mirror(W.brickwall(n, 20, seed=2024), barrier=True)
     b = HipSvBackend(seed=1)
     st = b.allocate(n)
     stats = b.run_program(st, c, jit=True)
+    assert stats["cancelled_gates"] == 0 and stats["n_passes"] >= 30
     f = _mirror_fidelity_lower_bound(b, st, n)
     norm = b.norm_sqr(st)
     st.free()
@@ -268,10 +269,11 @@ def test_mirror_headline_plan_n30_fp64():
 
 def test_mirror_depth40_n32_fp32():
     n = 32
-    c = mirror(W.brickwall(n, 40, seed=2024))
+    c = mirror(W.brickwall(n, 40, seed=2024), barrier=True)
     b = HipSvBackend(seed=1, precision=32)
     st = b.allocate(n)
     stats = b.run_program(st, c, jit=True)
+    assert stats["cancelled_gates"] == 0 and stats["n_passes"] >= 60
     f = _mirror_fidelity_lower_bound(b, st, n)
     norm = b.norm_sqr(st)
     st.free()

@@ -60,12 +60,17 @@ def gpu_gatewise(hip, circuit, psi0=None):
     return a
 
 
-def mirror(circuit):
-    """circuit followed by its inverse (U3(θ,φ,λ)^-1 = U3(−θ,−λ,−φ), kinds.rs inverse())."""
+def mirror(circuit, barrier: bool = False):
+    """circuit followed by its inverse (U3(θ,φ,λ)^-1 = U3(−θ,−λ,−φ), kinds.rs inverse()).
+    With `barrier`, a barrier on every qubit separates the halves, so the
+    planner's inverse-pair cancellation cannot collapse the mirror and every
+    gate is executed."""
     m = Circuit(circuit.num_qubits)
     gates = circuit.gates()
     for g in gates:
         m.gate(g.gate, g.qubits, g.controls)
+    if barrier:
+        m.barrier(range(circuit.num_qubits))
     for g in reversed(gates):
         gt = g.gate
         if gt.name == "U3":
@@ -240,9 +245,10 @@ def test_program_determinism(hip):
 
 def test_large_mirror_and_norm(hip):
     n = 26
-    c = mirror(W.brickwall(n, 6, seed=9))
+    c = mirror(W.brickwall(n, 6, seed=9), barrier=True)
     st = hip.allocate(n)
-    hip.run_program(st, c)
+    stats = hip.run_program(st, c)
+    assert stats["cancelled_gates"] == 0
     assert abs(hip.norm_sqr(st) - 1.0) < 1e-12
     amp0 = hip.probabilities(st, [])  # validates the state
     assert amp0[0] == 1.0
